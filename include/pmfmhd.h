@@ -1,0 +1,125 @@
+/* pmfmhd — GLM-HLLD finite-volume MHD step on MI355X (gfx950), C ABI.
+ *
+ * A drop-in for the step/flux API of yipihey/pmf-glm lib/MHDKernels: the state
+ * container, the per-step update, the CFL reduction, the diagnostics and the
+ * face-staggered constrained-transport (CT) variant. Every entry point returns an int
+ * status (0 = ok, nonzero = error); pmf_last_error() gives the calling thread's message,
+ * which reuses the reference's text where the same misuse is reported there.
+ *
+ * Layout (state.jl:1-13,95): each field is a flat array of nx*ny*nz values, cell
+ * (i,j,k) (0-based) at (k*ny + j)*nx + i. Conserved variables are numbered 0..8 in the
+ * reference's order (rho, mx, my, mz, E, Bx, By, Bz, psi). Buffers live on the GPU and
+ * are owned by the library; host pointers passed in are plain host memory.
+ * Calls on one state are not re-entrant; every call that returns a host value waits
+ * for the state's stream, the others are asynchronous on that stream.
+ */
+#ifndef PMFMHD_H
+#define PMFMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Codes, numbered as in the reference. */
+enum { PMF_RIEMANN_HLLD = 0, PMF_RIEMANN_HLL = 1, PMF_RIEMANN_LLF = 2 };          /* glm_physics.jl:309-311 */
+enum { PMF_RECON_PLM = 0, PMF_RECON_PPM = 1, PMF_RECON_PLM_MINMOD = 2 };         /* glm_physics.jl:202-204 */
+enum { PMF_BC_PERIODIC = 0, PMF_BC_OUTFLOW = 1, PMF_BC_REFLECTING = 2 };         /* boundary.jl:7-9 */
+enum { PMF_SOLVER_GLM = 0, PMF_SOLVER_CT = 1 };                                  /* state.jl:130-131 */
+enum { PMF_F32 = 0, PMF_F64 = 1 };
+enum { PMF_INTEGRATOR_REF = 0, PMF_INTEGRATOR_CUBE = 1 };                        /* driver.jl:25-35 */
+
+/* Keyword arguments of allocate_state (state.jl:123-129). */
+typedef struct pmf_params {
+  int32_t nx, ny, nz;
+  int32_t riemann;   /* PMF_RIEMANN_* */
+  int32_t recon;     /* PMF_RECON_* */
+  int32_t bc[3];     /* PMF_BC_* per axis */
+  int32_t solver;    /* PMF_SOLVER_* */
+  int32_t precision; /* PMF_F32 / PMF_F64: element type of every field */
+  double dx, gamma, smallr, pfl, llf_dmin, llf_pmin;
+} pmf_params;
+
+typedef struct pmf_state pmf_state;
+
+/* Fills the reference defaults: gamma 5/3, smallr 1e-6, pfl 1e-7, llf_dmin 1e-4,
+ * llf_pmin 0, HLLD, PLM (MonCen), periodic, GLM, f32 (state.jl:123-129). */
+int pmf_params_default(pmf_params* p, int32_t nx, int32_t ny, int32_t nz, double dx);
+
+/* allocate_state (state.jl:123-141): zero-filled U, scratch and, for CT, the 21-slot
+ * face/workspace buffer (state.jl:44-58), on the current HIP device. */
+int pmf_create(const pmf_params* p, pmf_state** out);
+int pmf_destroy(pmf_state* s);
+
+/* Host <-> device copies of one conserved variable (n = number of cells; the host
+ * buffer holds elements of the state's precision). fields_to_host: state.jl:144. */
+int pmf_upload(pmf_state* s, int32_t var, const void* host, int64_t n);
+int pmf_download(pmf_state* s, int32_t var, void* host, int64_t n);
+/* Device address of U[var] / scratch[var] for zero-copy wrapping. After pmf_step the
+ * scratch set holds the previous state (integrator_ref.jl:217); between steps scratch
+ * is caller workspace: the library keeps nothing there. */
+int pmf_device_ptr(pmf_state* s, int32_t var, void** ptr);
+int pmf_scratch_ptr(pmf_state* s, int32_t var, void** ptr);
+/* CT slot q in 0..20 (state.jl:17-37 order: bxf byf bzf, mfx mfy mfz, fxby fxbz fybx
+ * fybz fzbx fzby, ecx ecy ecz, ex ey ez, rbx rby rbz). */
+int pmf_ct_ptr(pmf_state* s, int32_t slot, void** ptr);
+/* Host copy of CT slot q (n = number of cells). */
+int pmf_ct_download(pmf_state* s, int32_t slot, void* host, int64_t n);
+
+/* Work on a caller stream (hipStream_t, default priority); NULL restores the state's own. */
+int pmf_set_stream(pmf_state* s, void* hip_stream);
+int pmf_synchronize(pmf_state* s);
+
+/* max_wavespeed! / compute_dt! (timestep.jl:17-41): smax = max over cells and axes of
+ * |v_d| + c_fast,d; dt = cfl*dx/max(smax, eps(T)), computed in the state's precision. */
+int pmf_max_wavespeed(pmf_state* s, double* smax);
+int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax);
+
+/* step_ref! / step_cube! (integrator_ref.jl:210-220, integrator_cube.jl:455-463): one
+ * MUSCL-Hancock step of size dt with GLM speed ch and psi factor decay, written to
+ * scratch and swapped in; CT states also run ct_update! (ct.jl:158-173) and ignore ch
+ * and decay. integrator: PMF_INTEGRATOR_REF (per-cell recompute) or
+ * PMF_INTEGRATOR_CUBE (fused tiled kernel; GLM only). */
+int pmf_step_raw(pmf_state* s, double dt, double ch, double decay, int32_t integrator);
+/* step! (driver.jl:20-37): decay = exp(-glm_cr*ch*dt/dx) in the state's precision. */
+int pmf_step(pmf_state* s, double dt, double ch, double glm_cr, int32_t integrator);
+/* evolve! (driver.jl:48-71) without a callback: CFL steps until t >= tfinal(1-1e-9),
+ * ch = glm_ch_fac*smax, dt scaled by smax/max(smax,ch), the last step clipped. */
+int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, double glm_cr,
+               int32_t integrator, double* t_out, int64_t* nsteps_out);
+
+/* Diagnostics. max_divb (diagnostics.jl:6-36): BC-aware centred |div B| maximum.
+ * pmf_divb_norm_fp64: the same divergence's float64 L2 norm sqrt(mean(d^2)) and max,
+ * with compensated summation. conserved_totals (diagnostics.jl:44-55): float64
+ * compensated sums x dV of rho, mx, my, mz, E, Bx, By, Bz (psi excluded). */
+int pmf_max_divb(pmf_state* s, double* out);
+int pmf_divb_norm_fp64(pmf_state* s, double* l2, double* linf);
+int pmf_conserved_totals_fp64(pmf_state* s, double out[8]);
+
+/* CT: max_divb_face (ct.jl:176-201); _ct_install_faces! (ct.jl:228-236: faces in,
+ * remainders zeroed, cell B = face average, E keeps the thermal pressure, psi = 0);
+ * ct_faces_from_potential! curl step (ct.jl:310-320) followed by the install. Host
+ * arrays of nx*ny*nz elements in the state's precision. */
+int pmf_max_divb_face(pmf_state* s, double* out);
+int pmf_ct_install_faces(pmf_state* s, const void* bxf, const void* byf, const void* bzf);
+int pmf_ct_faces_from_potential(pmf_state* s, const void* ax, const void* ay, const void* az);
+
+/* riemann / riemann_ct (glm_physics.jl:317-348) for n faces: L, R, F are host arrays of
+ * n x 9 primitive states / fluxes (row-major, state precision); dir 0,1,2 = x,y,z. */
+int pmf_riemann_flux_batch(pmf_state* s, const void* L, const void* R, int32_t dir, double ch,
+                           int64_t n, void* F);
+
+/* Kernel timing: while enabled, HIP events bracket every step-kernel launch on the
+ * state's stream; pmf_timing_read waits for the stream and returns the summed kernel
+ * time (ms) and the number of timed launches since the last enable. */
+int pmf_timing_enable(pmf_state* s, int32_t on);
+int pmf_timing_read(pmf_state* s, double* total_ms, int64_t* launches);
+
+const char* pmf_last_error(void);
+const char* pmf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMFMHD_H */

@@ -1,0 +1,577 @@
+// C ABI of the MI355X GLM-HLLD step (declared in include/pmfmhd.h).
+//
+// Host-side mirror of lib/MHDKernels: allocate_state (state.jl:123-141), step! /
+// evolve! (driver.jl:20-71), compute_dt! (timestep.jl:37-41), the diagnostics
+// (diagnostics.jl, ct.jl:176-201) and the CT set-up (ct.jl:228-236,332-345). Errors
+// reuse the reference's messages for the same misuse.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pmfmhd.h"
+#include "launch.hpp"
+
+using namespace pmf;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F> int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+}  // namespace
+
+struct pmf_state {
+  pmf_params p;
+  Grid G;
+  int64_t nc;
+  size_t esz;
+  void* U = nullptr;     // 9*nc
+  void* S = nullptr;     // 9*nc scratch
+  void* ct = nullptr;    // 21*nc (CT only)
+  double* red = nullptr; // reduction partials
+  double* dval = nullptr;  // device scalars: [0]=smax [1..3]=divb [4..11]=totals [12]=face div
+  double* hval = nullptr;  // pinned host mirror of dval
+  hipStream_t own = nullptr;
+  hipStream_t st = nullptr;
+  int device = 0;
+  bool smax_valid = false;  // dval[0] holds the CFL speed of the current U
+  bool timing = false;      // bracket step kernels with events
+  std::vector<hipEvent_t> ev;  // pairs (start, stop) in launch order
+  size_t ev_used = 0;
+};
+
+namespace {
+
+void check_state(const pmf_state* s) {
+  if (!s) throw Error("null pmf_state");
+}
+
+template <class T> StepParams<T> make_params(const pmf_state* s, double dt, double ch, double decay) {
+  const pmf_params& p = s->p;
+  StepParams<T> P;
+  P.gamma = T(p.gamma);
+  P.gm1 = T(p.gamma) - T(1);
+  P.igm1 = T(1) / P.gm1;
+  P.smallr = T(p.smallr);
+  P.pfl = T(p.pfl);
+  P.llf_dmin = T(p.llf_dmin);
+  P.llf_pmin = T(p.llf_pmin);
+  P.ch = T(ch);
+  P.ich = ch != 0.0 ? T(1) / T(ch) : T(0);
+  P.ch2 = T(ch) * T(ch);
+  P.dtdx = T(dt) / T(p.dx);
+  P.hdtdx = T(0.5) * P.dtdx;
+  P.decay = T(decay);
+  P.recon = p.recon;
+  P.rsolve = p.riemann;
+  return P;
+}
+
+void sync(pmf_state* s) { hip_check(hipStreamSynchronize(s->st), "hipStreamSynchronize"); }
+
+// Records one timing event on the state's stream (two per timed launch).
+void mark(pmf_state* s) {
+  if (!s->timing) return;
+  if (s->ev_used == s->ev.size()) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    s->ev.push_back(e);
+  }
+  hip_check(hipEventRecord(s->ev[s->ev_used++], s->st), "hipEventRecord");
+}
+
+void fetch(pmf_state* s, int first, int count) {
+  hip_check(hipMemcpyAsync(s->hval + first, s->dval + first, count * sizeof(double), hipMemcpyDeviceToHost, s->st),
+            "hipMemcpyAsync");
+  sync(s);
+}
+
+double smax_device(pmf_state* s) {
+  if (!s->smax_valid) {
+    if (s->p.precision == PMF_F64)
+      launch_wavespeed<double>((const double*)s->U, s->G, make_params<double>(s, 0, 0, 1), s->red, s->dval, s->st);
+    else
+      launch_wavespeed<float>((const float*)s->U, s->G, make_params<float>(s, 0, 0, 1), s->red, s->dval, s->st);
+    hip_check(hipGetLastError(), "wavespeed launch");
+    s->smax_valid = true;
+  }
+  fetch(s, 0, 1);
+  return s->hval[0];
+}
+
+// step_ref! / step_cube! with explicit (dt, ch, decay): kernel into scratch, swap, then
+// ct_update! for CT states (integrator_ref.jl:210-220).
+void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
+  const bool ct = s->p.solver == PMF_SOLVER_CT;
+  if (integrator != PMF_INTEGRATOR_REF && integrator != PMF_INTEGRATOR_CUBE)
+    throw Error("unknown integrator :" + std::to_string(integrator) + " (have :ref, :cube)");
+  if (ct && integrator == PMF_INTEGRATOR_CUBE)
+    throw Error("step!: CT is implemented on the :ref integrator only (mhd_solver=:ct states cannot use the :cube "
+                "integrators)");
+  if (!ct && !(ch > 0.0)) throw Error("step! needs the GLM cleaning speed `ch` (GLM states)");
+  bool done = false;
+  if (integrator == PMF_INTEGRATOR_CUBE) {
+    if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
+      throw Error("step_cube!: the fused kernel needs an all-periodic grid whose nx and ny are multiples of 16 "
+                  "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
+                  ")");
+    mark(s);
+    if (s->p.precision == PMF_F64)
+      done = launch_step_fused<double>((const double*)s->U, (double*)s->S, s->G,
+                                       make_params<double>(s, dt, ch, decay), s->red, s->dval, s->st);
+    else
+      done = launch_step_fused<float>((const float*)s->U, (float*)s->S, s->G, make_params<float>(s, dt, ch, decay),
+                                      s->red, s->dval, s->st);
+    mark(s);
+    if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
+    s->smax_valid = true;  // the fused epilogue reduced the new state's CFL speed
+  } else {
+    mark(s);
+    if (s->p.precision == PMF_F64)
+      launch_step_ref<double>((const double*)s->U, (double*)s->S, (double*)s->ct, s->G,
+                              make_params<double>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
+    else
+      launch_step_ref<float>((const float*)s->U, (float*)s->S, (float*)s->ct, s->G,
+                             make_params<float>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
+    mark(s);
+    s->smax_valid = false;
+  }
+  hip_check(hipGetLastError(), "step launch");
+  std::swap(s->U, s->S);
+  if (ct) {
+    if (s->p.precision == PMF_F64)
+      launch_ct_update<double>((double*)s->U, (double*)s->ct, s->G, double(dt) / s->p.dx, false, s->st);
+    else
+      launch_ct_update<float>((float*)s->U, (float*)s->ct, s->G, float(dt) / float(s->p.dx), false, s->st);
+    hip_check(hipGetLastError(), "ct_update launch");
+    s->smax_valid = false;
+  }
+}
+
+// decay = exp(-T(glm_cr)*T(ch)*T(dt)/dx) in the state's precision (driver.jl:29).
+double decay_of(const pmf_state* s, double dt, double ch, double glm_cr) {
+  if (s->p.precision == PMF_F64) return std::exp(-glm_cr * ch * dt / s->p.dx);
+  float v = -float(glm_cr) * float(ch) * float(dt) / float(s->p.dx);
+  return double(std::exp(v));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmf_params_default(pmf_params* p, int32_t nx, int32_t ny, int32_t nz, double dx) {
+  return guard([&] {
+    if (!p) throw Error("null params");
+    std::memset(p, 0, sizeof(*p));
+    p->nx = nx;
+    p->ny = ny;
+    p->nz = nz;
+    p->dx = dx;
+    p->gamma = 5.0 / 3.0;
+    p->smallr = 1e-6;
+    p->pfl = 1e-7;
+    p->llf_dmin = 1e-4;
+    p->llf_pmin = 0.0;
+    p->riemann = PMF_RIEMANN_HLLD;
+    p->recon = PMF_RECON_PLM;
+    p->solver = PMF_SOLVER_GLM;
+    p->precision = PMF_F32;
+  });
+}
+
+int pmf_create(const pmf_params* pp, pmf_state** out) {
+  return guard([&] {
+    if (!pp || !out) throw Error("null argument");
+    const pmf_params& p = *pp;
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1) throw Error("grid dimensions must be positive");
+    if (!(p.dx > 0)) throw Error("dx must be positive");
+    if (p.riemann < 0 || p.riemann > 2) throw Error("unknown MHD riemann code (have :hlld, :hll, :llf)");
+    if (p.recon < 0 || p.recon > 2) throw Error("unknown recon code (have :plm, :plm_minmod, :ppm)");
+    for (int d = 0; d < 3; ++d)
+      if (p.bc[d] < 0 || p.bc[d] > 2) throw Error("unknown BC code (have :periodic, :outflow, :reflecting)");
+    if (p.solver != PMF_SOLVER_GLM && p.solver != PMF_SOLVER_CT) throw Error("unknown mhd_solver (have :glm, :ct)");
+    if (p.precision != PMF_F32 && p.precision != PMF_F64) throw Error("precision must be PMF_F32 or PMF_F64");
+    if (p.solver == PMF_SOLVER_CT && (p.bc[0] || p.bc[1] || p.bc[2]))
+      throw Error("mhd_solver=:ct requires all-periodic boundaries; constrained transport is periodic-only");
+    auto* s = new pmf_state();
+    try {
+      s->p = p;
+      s->nc = int64_t(p.nx) * p.ny * p.nz;
+      s->G = Grid{p.nx, p.ny, p.nz, {p.bc[0], p.bc[1], p.bc[2]}, s->nc};
+      s->esz = p.precision == PMF_F64 ? 8 : 4;
+      hip_check(hipGetDevice(&s->device), "hipGetDevice");
+      hip_check(hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking), "hipStreamCreate");
+      s->st = s->own;
+      size_t bytes = 9 * s->nc * s->esz;
+      hip_check(hipMalloc(&s->U, bytes), "hipMalloc(U)");
+      hip_check(hipMalloc(&s->S, bytes), "hipMalloc(scratch)");
+      hip_check(hipMemsetAsync(s->U, 0, bytes, s->st), "hipMemset");
+      hip_check(hipMemsetAsync(s->S, 0, bytes, s->st), "hipMemset");
+      if (p.solver == PMF_SOLVER_CT) {
+        hip_check(hipMalloc(&s->ct, 21 * s->nc * s->esz), "hipMalloc(ct)");
+        hip_check(hipMemsetAsync(s->ct, 0, 21 * s->nc * s->esz, s->st), "hipMemset");
+      }
+      hip_check(hipMalloc(&s->red, sizeof(double) * kRedBlocks * kRedSlots), "hipMalloc(red)");
+      hip_check(hipMalloc(&s->dval, sizeof(double) * 64), "hipMalloc(dval)");
+      hip_check(hipHostMalloc((void**)&s->hval, sizeof(double) * 64, hipHostMallocDefault), "hipHostMalloc");
+      sync(s);
+    } catch (...) {
+      pmf_destroy(s);
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int pmf_destroy(pmf_state* s) {
+  return guard([&] {
+    if (!s) return;
+    if (s->own) (void)hipStreamSynchronize(s->own);
+    if (s->st && s->st != s->own) (void)hipStreamSynchronize(s->st);
+    (void)hipFree(s->U);
+    (void)hipFree(s->S);
+    (void)hipFree(s->ct);
+    (void)hipFree(s->red);
+    (void)hipFree(s->dval);
+    if (s->hval) (void)hipHostFree(s->hval);
+    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    if (s->own) (void)hipStreamDestroy(s->own);
+    delete s;
+  });
+}
+
+int pmf_upload(pmf_state* s, int32_t var, const void* host, int64_t n) {
+  return guard([&] {
+    check_state(s);
+    if (var < 0 || var > 8) throw Error("variable index must be 0..8");
+    if (n != s->nc) throw Error("host array has " + std::to_string(n) + " entries, expected " + std::to_string(s->nc));
+    hip_check(hipMemcpyAsync((char*)s->U + var * s->nc * s->esz, host, n * s->esz, hipMemcpyHostToDevice, s->st),
+              "upload");
+    sync(s);
+    s->smax_valid = false;
+  });
+}
+
+int pmf_download(pmf_state* s, int32_t var, void* host, int64_t n) {
+  return guard([&] {
+    check_state(s);
+    if (var < 0 || var > 8) throw Error("variable index must be 0..8");
+    if (n != s->nc) throw Error("host array has " + std::to_string(n) + " entries, expected " + std::to_string(s->nc));
+    hip_check(hipMemcpyAsync(host, (char*)s->U + var * s->nc * s->esz, n * s->esz, hipMemcpyDeviceToHost, s->st),
+              "download");
+    sync(s);
+  });
+}
+
+int pmf_device_ptr(pmf_state* s, int32_t var, void** ptr) {
+  return guard([&] {
+    check_state(s);
+    if (var < 0 || var > 8 || !ptr) throw Error("variable index must be 0..8");
+    *ptr = (char*)s->U + var * s->nc * s->esz;
+    s->smax_valid = false;  // the caller may write through the pointer
+  });
+}
+
+int pmf_scratch_ptr(pmf_state* s, int32_t var, void** ptr) {
+  return guard([&] {
+    check_state(s);
+    if (var < 0 || var > 8 || !ptr) throw Error("variable index must be 0..8");
+    *ptr = (char*)s->S + var * s->nc * s->esz;
+  });
+}
+
+int pmf_ct_ptr(pmf_state* s, int32_t slot, void** ptr) {
+  return guard([&] {
+    check_state(s);
+    if (!s->ct) throw Error("ct_update! needs a CT state");
+    if (slot < 0 || slot > 20 || !ptr) throw Error("CT slot must be 0..20");
+    *ptr = (char*)s->ct + slot * s->nc * s->esz;
+    s->smax_valid = false;
+  });
+}
+
+int pmf_ct_download(pmf_state* s, int32_t slot, void* host, int64_t n) {
+  return guard([&] {
+    check_state(s);
+    if (!s->ct) throw Error("ct_update! needs a CT state");
+    if (slot < 0 || slot > 20) throw Error("CT slot must be 0..20");
+    if (n != s->nc) throw Error("host array has " + std::to_string(n) + " entries, expected " + std::to_string(s->nc));
+    hip_check(hipMemcpyAsync(host, (char*)s->ct + slot * s->nc * s->esz, n * s->esz, hipMemcpyDeviceToHost, s->st),
+              "download");
+    sync(s);
+  });
+}
+
+int pmf_set_stream(pmf_state* s, void* stream) {
+  return guard([&] {
+    check_state(s);
+    sync(s);
+    s->st = stream ? (hipStream_t)stream : s->own;
+  });
+}
+
+int pmf_synchronize(pmf_state* s) {
+  return guard([&] {
+    check_state(s);
+    sync(s);
+  });
+}
+
+int pmf_max_wavespeed(pmf_state* s, double* smax) {
+  return guard([&] {
+    check_state(s);
+    *smax = smax_device(s);
+  });
+}
+
+int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax) {
+  return guard([&] {
+    check_state(s);
+    double sm = smax_device(s);
+    if (s->p.precision == PMF_F64) {
+      *dt = cfl * s->p.dx / std::max(sm, std::numeric_limits<double>::epsilon());
+    } else {
+      float smf = float(sm);
+      *dt = double(float(cfl) * float(s->p.dx) / std::max(smf, std::numeric_limits<float>::epsilon()));
+    }
+    if (smax) *smax = sm;
+  });
+}
+
+int pmf_step_raw(pmf_state* s, double dt, double ch, double decay, int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    do_step(s, dt, ch, decay, integrator);
+  });
+}
+
+int pmf_step(pmf_state* s, double dt, double ch, double glm_cr, int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    if (s->p.solver == PMF_SOLVER_CT) {
+      do_step(s, dt, 0.0, 1.0, integrator);
+      return;
+    }
+    if (!(ch > 0.0)) throw Error("step! needs the GLM cleaning speed `ch` (GLM states)");
+    do_step(s, dt, ch, decay_of(s, dt, ch, glm_cr), integrator);
+  });
+}
+
+int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, double glm_cr, int32_t integrator,
+               double* t_out, int64_t* n_out) {
+  return guard([&] {
+    check_state(s);
+    if (!(glm_ch_fac > 0)) throw Error("glm_ch_fac must be positive");
+    if (!(glm_cr >= 0)) throw Error("glm_cr must be non-negative");
+    const bool ct = s->p.solver == PMF_SOLVER_CT;
+    const bool f64 = s->p.precision == PMF_F64;
+    double t = 0.0;
+    int64_t n = 0;
+    while (t < tfinal * (1 - 1e-9)) {
+      double dt, smax;
+      if (pmf_compute_dt(s, cfl, &dt, &smax)) throw Error(g_err);
+      double ch, dtf;
+      if (ct) {
+        ch = 0.0;
+        dtf = dt;
+      } else if (f64) {
+        ch = glm_ch_fac * smax;
+        dtf = dt * smax / std::max(smax, ch);
+      } else {
+        float chf = float(glm_ch_fac) * float(smax);
+        ch = chf;
+        dtf = dt * double(float(smax)) / double(std::max(float(smax), chf));
+      }
+      if (t + dtf > tfinal) dtf = tfinal - t;
+      if (ct) do_step(s, dtf, 0.0, 1.0, integrator);
+      else do_step(s, dtf, ch, decay_of(s, dtf, ch, glm_cr), integrator);
+      t += dtf;
+      ++n;
+    }
+    if (t_out) *t_out = t;
+    if (n_out) *n_out = n;
+  });
+}
+
+int pmf_max_divb(pmf_state* s, double* out) {
+  return guard([&] {
+    check_state(s);
+    double l2, linf;
+    if (pmf_divb_norm_fp64(s, &l2, &linf)) throw Error(g_err);
+    *out = linf;
+  });
+}
+
+int pmf_divb_norm_fp64(pmf_state* s, double* l2, double* linf) {
+  return guard([&] {
+    check_state(s);
+    if (s->p.precision == PMF_F64)
+      launch_divb<double>((const double*)s->U, s->G, 1.0 / (2.0 * s->p.dx), s->red, s->dval + 1, s->st);
+    else
+      launch_divb<float>((const float*)s->U, s->G, 1.0f / (2.0f * float(s->p.dx)), s->red, s->dval + 1, s->st);
+    hip_check(hipGetLastError(), "divb launch");
+    fetch(s, 1, 2);
+    if (l2) *l2 = std::sqrt(s->hval[1] / double(s->nc));
+    if (linf) *linf = s->hval[2];
+  });
+}
+
+int pmf_conserved_totals_fp64(pmf_state* s, double out[8]) {
+  return guard([&] {
+    check_state(s);
+    if (s->p.precision == PMF_F64) launch_totals<double>((const double*)s->U, s->G, s->red, s->dval + 4, s->st);
+    else launch_totals<float>((const float*)s->U, s->G, s->red, s->dval + 4, s->st);
+    hip_check(hipGetLastError(), "totals launch");
+    fetch(s, 4, 8);
+    double dV = s->p.dx * s->p.dx * s->p.dx;
+    for (int v = 0; v < 8; ++v) out[v] = s->hval[4 + v] * dV;
+  });
+}
+
+int pmf_max_divb_face(pmf_state* s, double* out) {
+  return guard([&] {
+    check_state(s);
+    if (!s->ct) throw Error("max_divb_face needs a CT state (mhd_solver=:ct)");
+    if (s->p.precision == PMF_F64)
+      launch_divb_face<double>((const double*)s->ct, s->G, 1.0 / s->p.dx, s->red, s->dval + 12, s->st);
+    else
+      launch_divb_face<float>((const float*)s->ct, s->G, 1.0f / float(s->p.dx), s->red, s->dval + 12, s->st);
+    hip_check(hipGetLastError(), "divb_face launch");
+    fetch(s, 12, 1);
+    *out = s->hval[12];
+  });
+}
+
+int pmf_ct_install_faces(pmf_state* s, const void* bx, const void* by, const void* bz) {
+  return guard([&] {
+    check_state(s);
+    if (!s->ct) throw Error("_ct_install_faces! needs a CT state (mhd_solver=:ct)");
+    size_t b = s->nc * s->esz;
+    const void* src[3] = {bx, by, bz};
+    for (int d = 0; d < 3; ++d)
+      hip_check(hipMemcpyAsync((char*)s->ct + d * b, src[d], b, hipMemcpyHostToDevice, s->st), "upload faces");
+    if (s->p.precision == PMF_F64) launch_ct_install<double>((double*)s->U, (double*)s->ct, s->G, s->st);
+    else launch_ct_install<float>((float*)s->U, (float*)s->ct, s->G, s->st);
+    hip_check(hipGetLastError(), "ct_install launch");
+    sync(s);
+    s->smax_valid = false;
+  });
+}
+
+int pmf_ct_faces_from_potential(pmf_state* s, const void* ax, const void* ay, const void* az) {
+  return guard([&] {
+    check_state(s);
+    if (!s->ct) throw Error("ct_faces_from_potential! needs a CT state (mhd_solver=:ct)");
+    size_t b = s->nc * s->esz;
+    // the potential is staged in scratch slots 0..2 (caller workspace between steps)
+    const void* src[3] = {ax, ay, az};
+    for (int d = 0; d < 3; ++d)
+      hip_check(hipMemcpyAsync((char*)s->S + d * b, src[d], b, hipMemcpyHostToDevice, s->st), "upload potential");
+    char* S = (char*)s->S;
+    char* F = (char*)s->ct;
+    if (s->p.precision == PMF_F64)
+      launch_ct_curl<double>((double*)S, (double*)(S + b), (double*)(S + 2 * b), (double*)F, (double*)(F + b),
+                             (double*)(F + 2 * b), s->G, 1.0 / s->p.dx, s->st);
+    else
+      launch_ct_curl<float>((float*)S, (float*)(S + b), (float*)(S + 2 * b), (float*)F, (float*)(F + b),
+                            (float*)(F + 2 * b), s->G, 1.0f / float(s->p.dx), s->st);
+    if (s->p.precision == PMF_F64) launch_ct_install<double>((double*)s->U, (double*)s->ct, s->G, s->st);
+    else launch_ct_install<float>((float*)s->U, (float*)s->ct, s->G, s->st);
+    hip_check(hipGetLastError(), "ct curl launch");
+    sync(s);
+    s->smax_valid = false;
+  });
+}
+
+int pmf_riemann_flux_batch(pmf_state* s, const void* L, const void* R, int32_t dir, double ch, int64_t n, void* F) {
+  return guard([&] {
+    check_state(s);
+    if (dir < 0 || dir > 2) throw Error("dir must be 0, 1 or 2");
+    if (n < 0) throw Error("n must be non-negative");
+    if (n == 0) return;
+    const bool ct = s->p.solver == PMF_SOLVER_CT;
+    if (!ct && !(ch > 0.0)) throw Error("riemann needs the GLM cleaning speed `ch` (GLM states)");
+    size_t b = 9 * n * s->esz;
+    void *dL = nullptr, *dR = nullptr, *dF = nullptr;
+    try {
+      hip_check(hipMalloc(&dL, b), "hipMalloc");
+      hip_check(hipMalloc(&dR, b), "hipMalloc");
+      hip_check(hipMalloc(&dF, b), "hipMalloc");
+      hip_check(hipMemcpyAsync(dL, L, b, hipMemcpyHostToDevice, s->st), "copy");
+      hip_check(hipMemcpyAsync(dR, R, b, hipMemcpyHostToDevice, s->st), "copy");
+      if (s->p.precision == PMF_F64)
+        launch_riemann_batch<double>((const double*)dL, (const double*)dR, (double*)dF, n, dir, ct,
+                                     make_params<double>(s, 0, ct ? 0.0 : ch, 1), s->st);
+      else
+        launch_riemann_batch<float>((const float*)dL, (const float*)dR, (float*)dF, n, dir, ct,
+                                    make_params<float>(s, 0, ct ? 0.0 : ch, 1), s->st);
+      hip_check(hipGetLastError(), "riemann launch");
+      hip_check(hipMemcpyAsync(F, dF, b, hipMemcpyDeviceToHost, s->st), "copy");
+      sync(s);
+    } catch (...) {
+      (void)hipFree(dL);
+      (void)hipFree(dR);
+      (void)hipFree(dF);
+      throw;
+    }
+    (void)hipFree(dL);
+    (void)hipFree(dR);
+    (void)hipFree(dF);
+  });
+}
+
+int pmf_timing_enable(pmf_state* s, int32_t on) {
+  return guard([&] {
+    check_state(s);
+    sync(s);
+    s->timing = on != 0;
+    s->ev_used = 0;
+  });
+}
+
+int pmf_timing_read(pmf_state* s, double* total_ms, int64_t* launches) {
+  return guard([&] {
+    check_state(s);
+    sync(s);
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < s->ev_used; i += 2) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "hipEventElapsedTime");
+      tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = int64_t(s->ev_used / 2);
+  });
+}
+
+const char* pmf_last_error(void) { return g_err.c_str(); }
+
+const char* pmf_version(void) { return "pmfmhd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+// Device-side physics of the GLM-MHD MUSCL-Hancock step, for gfx950.
+//
+// Same scheme as lib/MHDKernels/src/glm_physics.jl (cited per function), organised for
+// the GPU: fluxes are evaluated in the face-normal frame through compile-time component
+// maps instead of permuted copies, reciprocals and reciprocal square roots are shared
+// between the terms that divide by the same quantity, and the four-way HLLD flux choice
+// is a select between the left and right star fans rather than four flux evaluations.
+// Float kernels use the hardware v_rcp_f32 / v_sqrt_f32 / v_rsq_f32 (about 1 ulp);
+// double kernels use exact division and square root.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pmf {
+
+enum { RECON_PLM = 0, RECON_PPM = 1, RECON_PLM_MINMOD = 2 };
+enum { RS_HLLD = 0, RS_HLL = 1, RS_LLF = 2 };
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECT = 2 };
+
+// Scalars of one step, prepared on the host in the state's precision.
+template <class T> struct StepParams {
+  T gamma, gm1, igm1;        // gamma, gamma-1, 1/(gamma-1)
+  T smallr, pfl, llf_dmin, llf_pmin;
+  T ch, ich, ch2;            // GLM speed, 1/ch, ch^2 (ch = 0 for CT: ich unused)
+  T dtdx, hdtdx, decay;      // dt/dx, dt/(2dx), psi factor
+  int32_t recon, rsolve;
+};
+
+struct Grid {
+  int32_t nx, ny, nz;
+  int32_t bc[3];
+  int64_t nc;
+};
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq(double x) { return 1.0 / __builtin_sqrt(x); }
+__device__ __forceinline__ float dmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float dmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float dabs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double dabs(double a) { return __builtin_fabs(a); }
+
+// cons2prim (glm_physics.jl:11-18): density floored before the velocity and kinetic
+// energy, pressure floored at pfl.
+template <class T> __device__ __forceinline__ void cons2prim(const T* c, T* q, const StepParams<T>& P) {
+  T rho = dmax(c[0], P.smallr);
+  T ir = rcp(rho);
+  T ek = T(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
+  T em = T(0.5) * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
+  q[0] = rho;
+  q[1] = c[1] * ir;
+  q[2] = c[2] * ir;
+  q[3] = c[3] * ir;
+  q[4] = dmax(P.gm1 * (c[4] - ek - em), P.pfl);
+  q[5] = c[5];
+  q[6] = c[6];
+  q[7] = c[7];
+  q[8] = c[8];
+}
+
+// prim2cons (glm_physics.jl:19-23).
+template <class T> __device__ __forceinline__ void prim2cons(const T* q, T* u, const StepParams<T>& P) {
+  T rho = q[0];
+  u[0] = rho;
+  u[1] = rho * q[1];
+  u[2] = rho * q[2];
+  u[3] = rho * q[3];
+  u[4] = q[4] * P.igm1 + T(0.5) * rho * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) +
+         T(0.5) * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
+  u[5] = q[5];
+  u[6] = q[6];
+  u[7] = q[7];
+  u[8] = q[8];
+}
+
+// fast_speed (glm_physics.jl:24-27) from density reciprocal ir, pressure p, |B|^2 b2 and
+// normal field bn.
+template <class T> __device__ __forceinline__ T cfast(T ir, T p, T b2, T bn, T gamma) {
+  T c2 = gamma * p * ir;
+  T d2 = T(0.5) * (b2 * ir + c2);
+  return dsqrt(d2 + dsqrt(dmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
+}
+
+// Physical flux along axis D (glm_physics.jl:28-34 composed with the cyclic rotations
+// of :36-45 and dir_flux :61): normal n = D, transverse a = D+1, b = D+2 (mod 3). The
+// normal-B and psi slots are zero.
+template <int D, class T> __device__ __forceinline__ void dflux(const T* q, T* f, T igm1) {
+  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
+  T rho = q[0], un = q[1 + N], ua = q[1 + A], ub = q[1 + B], p = q[4];
+  T bn = q[5 + N], ba = q[5 + A], bb = q[5 + B];
+  T b2 = bn * bn + ba * ba + bb * bb;
+  T pt = p + T(0.5) * b2;
+  T E = p * igm1 + T(0.5) * rho * (un * un + ua * ua + ub * ub) + T(0.5) * b2;
+  T vb = un * bn + ua * ba + ub * bb;
+  T m = rho * un;
+  f[0] = m;
+  f[1 + N] = m * un + pt - bn * bn;
+  f[1 + A] = m * ua - bn * ba;
+  f[1 + B] = m * ub - bn * bb;
+  f[4] = (E + pt) * un - bn * vb;
+  f[5 + N] = T(0);
+  f[5 + A] = un * ba - ua * bn;
+  f[5 + B] = un * bb - ub * bn;
+  f[8] = T(0);
+}
+
+// Limited face offsets (dL, dR) of one primitive from its two neighbours
+// (glm_physics.jl:47-59 PLM MonCen / minmod, 188-200 PPM, 212-223).
+template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0, T qp, T& dL, T& dR) {
+  if constexpr (RECON == RECON_PPM) {
+    T slope = (qp - qm) * T(0.25);
+    T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
+    T qL = q0 - slope + curve, qR = q0 + slope + curve;
+    T dq = qR - qL;
+    T diff = (q0 - T(0.5) * (qL + qR)) * dq;
+    T lim = dq * dq * T(1.0 / 6.0);
+    bool flat = (qR - q0) * (q0 - qL) <= T(0);
+    T eL = flat ? q0 : (diff > lim ? T(3) * q0 - T(2) * qR : qL);
+    T eR = flat ? q0 : ((diff <= lim && diff < -lim) ? T(3) * q0 - T(2) * qL : qR);
+    dL = eL - q0;
+    dR = eR - q0;
+  } else {
+    T dl = q0 - qm, dr = qp - q0;
+    T s;
+    if constexpr (RECON == RECON_PLM_MINMOD) {
+      T m = dmin(dabs(dl), dabs(dr));
+      s = dl >= T(0) ? m : -m;
+    } else {
+      T dc = T(0.5) * (dl + dr);
+      T m = dmin(T(2) * dmin(dabs(dl), dabs(dr)), dabs(dc));
+      s = dc >= T(0) ? m : -m;
+    }
+    s = dl * dr <= T(0) ? T(0) : T(0.5) * s;
+    dL = -s;
+    dR = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers in the face-normal frame. States are primitive 9-vectors ordered
+// (rho, un, ua, ub, p, bn, ba, bb, psi); fluxes come back in the same frame.
+
+template <class T>
+__device__ __forceinline__ void llf_n(const T* L, const T* R, T ch, T fbn, T fpsi, T* f, const StepParams<T>& P) {
+  // glm_physics.jl:239-244
+  T irL = rcp(L[0]), irR = rcp(R[0]);
+  T b2L = L[5] * L[5] + L[6] * L[6] + L[7] * L[7];
+  T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
+  T smax = dmax(dmax(dabs(L[1]) + cfast(irL, L[4], b2L, L[5], P.gamma),
+                     dabs(R[1]) + cfast(irR, R[4], b2R, R[5], P.gamma)), ch);
+  T fL[9], fR[9], uL[9], uR[9];
+  dflux<0>(L, fL, P.igm1);
+  dflux<0>(R, fR, P.igm1);
+  prim2cons(L, uL, P);
+  prim2cons(R, uR, P);
+  T hs = T(0.5) * smax;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f[i] = T(0.5) * (fL[i] + fR[i]) - hs * (uR[i] - uL[i]);
+  f[5] = fbn;
+  f[8] = fpsi;
+}
+
+template <class T>
+__device__ __forceinline__ void hll_n(const T* L, const T* R, T bn, T fbn, T fpsi, T* f, const StepParams<T>& P) {
+  // glm_physics.jl:245-252
+  T irL = rcp(L[0]), irR = rcp(R[0]);
+  T cfL = cfast(irL, L[4], L[5] * L[5] + L[6] * L[6] + L[7] * L[7], bn, P.gamma);
+  T cfR = cfast(irR, R[4], R[5] * R[5] + R[6] * R[6] + R[7] * R[7], bn, P.gamma);
+  T SL = dmin(dmin(L[1] - cfL, R[1] - cfR), T(0));
+  T SR = dmax(dmax(L[1] + cfL, R[1] + cfR), T(0));
+  T fL[9], fR[9], uL[9], uR[9];
+  dflux<0>(L, fL, P.igm1);
+  dflux<0>(R, fR, P.igm1);
+  prim2cons(L, uL, P);
+  prim2cons(R, uR, P);
+  T inv = rcp(SR - SL);
+  T sLR = SL * SR;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f[i] = (SR * fL[i] - SL * fR[i] + sLR * (uR[i] - uL[i])) * inv;
+  f[5] = fbn;
+  f[8] = fpsi;
+}
+
+// HLLD (Miyoshi & Kusano 2005) as in glm_physics.jl:255-307. The flux is
+//   F_side + S*(U*_side - U_side) + [inner]*S*_side*(U**_side - U*_side)
+// with side = left when SM >= 0 and "inner" the two branches that use the double-star
+// state; this equals the reference's four formulas branch by branch.
+template <class T>
+__device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fpsi, T* f, const StepParams<T>& P) {
+  const T dL = L[0], uL = L[1], vL = L[2], wL = L[3], pL = L[4], byL = L[6], bzL = L[7];
+  const T dR = R[0], uR = R[1], vR = R[2], wR = R[3], pR = R[4], byR = R[6], bzR = R[7];
+  const T bn2 = bn * bn;
+  T irL = rcp(dL), irR = rcp(dR);
+  T b2L = bn2 + byL * byL + bzL * bzL, b2R = bn2 + byR * byR + bzR * bzR;
+  T ptL = pL + T(0.5) * b2L, ptR = pR + T(0.5) * b2R;
+  T EL = pL * P.igm1 + T(0.5) * dL * (uL * uL + vL * vL + wL * wL) + T(0.5) * b2L;
+  T ER = pR * P.igm1 + T(0.5) * dR * (uR * uR + vR * vR + wR * wR) + T(0.5) * b2R;
+  T cfL = cfast(irL, pL, b2L, bn, P.gamma), cfR = cfast(irR, pR, b2R, bn, P.gamma);
+  T cmax = dmax(cfL, cfR);
+  T SL = dmin(dmin(uL, uR) - cmax, T(0));
+  T SR = dmax(dmax(uL, uR) + cmax, T(0));
+  T aL = (SL - uL) * dL, aR = (SR - uR) * dR;   // mass fluxes relative to the outer waves
+  T iden = rcp(aR - aL);
+  T SM = (aR * uR - aL * uL - ptR + ptL) * iden;
+  T pts = (aR * ptL - aL * ptR + aL * aR * (uR - uL)) * iden;
+  T iLM = rcp(SL - SM), iRM = rcp(SR - SM);
+  T dLs = dmax(aL * iLM, T(1e-12)), dRs = dmax(aR * iRM, T(1e-12));
+  T rqL = rsq(dLs), rqR = rsq(dRs);
+  T sqL = dLs * rqL, sqR = dRs * rqR;
+  T abn = dabs(bn);
+  T SLs = SM - abn * rqL, SRs = SM + abn * rqR;
+  // single-star transverse states (glm_physics.jl:255-263)
+  T vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
+  {
+    T denK = aL * (SL - SM) - bn2;
+    bool deg = dabs(denK) < T(1e-12);
+    T ik = rcp(denK);
+    T fac = bn * (SM - uL) * ik;
+    T gg = (aL * (SL - uL) - bn2) * ik;
+    vyLs = deg ? vL : vL - byL * fac;
+    vzLs = deg ? wL : wL - bzL * fac;
+    byLs = deg ? byL : byL * gg;
+    bzLs = deg ? bzL : bzL * gg;
+  }
+  {
+    T denK = aR * (SR - SM) - bn2;
+    bool deg = dabs(denK) < T(1e-12);
+    T ik = rcp(denK);
+    T fac = bn * (SM - uR) * ik;
+    T gg = (aR * (SR - uR) - bn2) * ik;
+    vyRs = deg ? vR : vR - byR * fac;
+    vzRs = deg ? wR : wR - bzR * fac;
+    byRs = deg ? byR : byR * gg;
+    bzRs = deg ? bzR : bzR * gg;
+  }
+  T vbL = uL * bn + vL * byL + wL * bzL, vbLs = SM * bn + vyLs * byLs + vzLs * bzLs;
+  T vbR = uR * bn + vR * byR + wR * bzR, vbRs = SM * bn + vyRs * byRs + vzRs * bzRs;
+  T ELs = ((SL - uL) * EL - ptL * uL + pts * SM + bn * (vbL - vbLs)) * iLM;
+  T ERs = ((SR - uR) * ER - ptR * uR + pts * SM + bn * (vbR - vbRs)) * iRM;
+  // double-star state (glm_physics.jl:288-296)
+  T sg = bn >= T(0) ? T(1) : T(-1);
+  T isum = rcp(sqL + sqR);
+  T vyss = (sqL * vyLs + sqR * vyRs + (byRs - byLs) * sg) * isum;
+  T vzss = (sqL * vzLs + sqR * vzRs + (bzRs - bzLs) * sg) * isum;
+  T byss = (sqL * byRs + sqR * byLs + sqL * sqR * (vyRs - vyLs) * sg) * isum;
+  T bzss = (sqL * bzRs + sqR * bzLs + sqL * sqR * (vzRs - vzLs) * sg) * isum;
+  T vbss = SM * bn + vyss * byss + vzss * bzss;
+  // side selection (glm_physics.jl:297-305)
+  const bool left = SM >= T(0);
+  const T d = left ? dL : dR, u = left ? uL : uR, v = left ? vL : vR, w = left ? wL : wR;
+  const T p = left ? pL : pR, by = left ? byL : byR, bz = left ? bzL : bzR;
+  const T E = left ? EL : ER, pt = left ? ptL : ptR;
+  const T S = left ? SL : SR, Ss = left ? SLs : SRs;
+  const T ds = left ? dLs : dRs, vys = left ? vyLs : vyRs, vzs = left ? vzLs : vzRs;
+  const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs, Es = left ? ELs : ERs;
+  const T sq = left ? sqL : sqR, vbs = left ? vbLs : vbRs;
+  const T Ess = left ? Es - sq * (vbs - vbss) * sg : Es + sq * (vbs - vbss) * sg;
+  const bool inner = left ? (SLs < T(0)) : (SRs >= T(0));
+  const T a = inner ? Ss : T(0);
+  // side physical flux (phys_flux_x of the state with normal field bn)
+  T b2 = left ? b2L : b2R;
+  T m = d * u;
+  T vb = u * bn + v * by + w * bz;
+  T F0 = m, F1 = m * u + pt - bn2, F2 = m * v - bn * by, F3 = m * w - bn * bz;
+  T F4 = (p * P.igm1 + T(0.5) * d * (u * u + v * v + w * w) + T(0.5) * b2 + pt) * u - bn * vb;
+  T F6 = u * by - v * bn, F7 = u * bz - w * bn;
+  f[0] = F0 + S * (ds - d);
+  f[1] = F1 + S * (ds * SM - m);
+  f[2] = F2 + S * (ds * vys - d * v) + a * (ds * (vyss - vys));
+  f[3] = F3 + S * (ds * vzs - d * w) + a * (ds * (vzss - vzs));
+  f[4] = F4 + S * (Es - E) + a * (Ess - Es);
+  f[5] = fbn;
+  f[6] = F6 + S * (bys - by) + a * (byss - bys);
+  f[7] = F7 + S * (bzs - bz) + a * (bzss - bzs);
+  f[8] = fpsi;
+}
+
+// riemann / riemann_ct (glm_physics.jl:317-348) on normal-frame states (modified in place:
+// floors, and the normal field replaced by the GLM b_n*, or by the CT face value).
+template <bool CT, class T>
+__device__ __forceinline__ void riemann_n(T* L, T* R, T* f, const StepParams<T>& P) {
+  L[0] = dmax(L[0], P.smallr);
+  L[4] = dmax(L[4], P.pfl);
+  R[0] = dmax(R[0], P.smallr);
+  R[4] = dmax(R[4], P.pfl);
+  T bns, fbn, fpsi, ch;
+  if constexpr (CT) {
+    bns = L[5];
+    fbn = T(0);
+    fpsi = T(0);
+    ch = T(0);
+  } else {
+    // glm_pair (glm_physics.jl:234-238)
+    bns = T(0.5) * (L[5] + R[5]) - T(0.5) * (R[8] - L[8]) * P.ich;
+    fbn = T(0.5) * (L[8] + R[8]) - T(0.5) * P.ch * (R[5] - L[5]);
+    fpsi = P.ch2 * bns;
+    ch = P.ch;
+  }
+  L[5] = bns;
+  R[5] = bns;
+  if (P.rsolve == RS_LLF) {
+    llf_n(L, R, ch, fbn, fpsi, f, P);
+  } else if (P.rsolve == RS_HLL) {
+    hll_n(L, R, bns, fbn, fpsi, f, P);
+  } else {
+    bool uself = (P.llf_dmin > T(0) && dmin(L[0], R[0]) < P.llf_dmin) ||
+                 (P.llf_pmin > T(0) && dmin(L[4], R[4]) < P.llf_pmin);
+    if (uself)
+      llf_n(L, R, ch, fbn, fpsi, f, P);
+    else
+      hlld_n(L, R, bns, fbn, fpsi, f, P);
+  }
+}
+
+// Lab-frame face flux across an axis-D face from lab-frame primitive states.
+template <int D, bool CT, class T>
+__device__ __forceinline__ void face_flux(const T* Lq, const T* Rq, T* F, const StepParams<T>& P) {
+  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
+  T L[9] = {Lq[0], Lq[1 + N], Lq[1 + A], Lq[1 + B], Lq[4], Lq[5 + N], Lq[5 + A], Lq[5 + B], Lq[8]};
+  T R[9] = {Rq[0], Rq[1 + N], Rq[1 + A], Rq[1 + B], Rq[4], Rq[5 + N], Rq[5 + A], Rq[5 + B], Rq[8]};
+  T f[9];
+  riemann_n<CT>(L, R, f, P);
+  F[0] = f[0];
+  F[1 + N] = f[1];
+  F[1 + A] = f[2];
+  F[1 + B] = f[3];
+  F[4] = f[4];
+  F[5 + N] = f[5];
+  F[5 + A] = f[6];
+  F[5 + B] = f[7];
+  F[8] = f[8];
+}
+
+// Per-cell CFL speed max_d(|v_d| + c_fast,d) of a conserved state (timestep.jl:8-12).
+template <class T> __device__ __forceinline__ T cell_wavespeed(const T* c, const StepParams<T>& P) {
+  T q[9];
+  cons2prim(c, q, P);
+  T ir = rcp(q[0]);
+  T b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+  T sx = dabs(q[1]) + cfast(ir, q[4], b2, q[5], P.gamma);
+  T sy = dabs(q[2]) + cfast(ir, q[4], b2, q[6], P.gamma);
+  T sz = dabs(q[3]) + cfast(ir, q[4], b2, q[7], P.gamma);
+  return dmax(sx, dmax(sy, sz));
+}
+
+// NaN-propagating max (Julia's `maximum` returns NaN when any element is NaN).
+template <class T> __device__ __forceinline__ T nanmax(T a, T b) {
+  return (a != a || b != b) ? (a + b) : dmax(a, b);
+}
+
+}  // namespace pmf

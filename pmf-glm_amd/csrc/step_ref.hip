@@ -1,0 +1,247 @@
+// Per-cell GLM / CT step (the portable path, lib/MHDKernels/src/integrator_ref.jl:69-160):
+// one work-item per cell rebuilds the Hancock prediction of itself and of its six face
+// neighbours from a +-2 stencil and solves its six faces. Arbitrary per-axis boundary
+// conditions (boundary.jl:17-43) are synthesised at load time. Also: the Riemann-flux
+// batch entry used by unit tests and by the surrogate's training data.
+#include "launch.hpp"
+
+namespace pmf {
+
+// boundary.jl:17-31 (0-based): in-range source index and flip sign for one axis.
+__device__ __forceinline__ int bc_map(int i, int n, int code, float& flip) {
+  flip = 1.0f;
+  if (code == BC_PERIODIC) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+  }
+  if (code == BC_OUTFLOW) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+  if (i < 0) {
+    flip = -1.0f;
+    return -1 - i;
+  }
+  if (i >= n) {
+    flip = -1.0f;
+    return 2 * n - 1 - i;
+  }
+  return i;
+}
+
+template <class T>
+__device__ __forceinline__ void load_prim(const T* __restrict__ U, const Grid& G, int i, int j, int k, T* q,
+                                          const StepParams<T>& P) {
+  float fx, fy, fz;
+  int ii = bc_map(i, G.nx, G.bc[0], fx), jj = bc_map(j, G.ny, G.bc[1], fy), kk = bc_map(k, G.nz, G.bc[2], fz);
+  int64_t idx = ((int64_t)kk * G.ny + jj) * G.nx + ii;
+  const int64_t n = G.nc;
+  T c[9];
+  c[0] = U[idx];
+  c[1] = U[n + idx] * T(fx);
+  c[2] = U[2 * n + idx] * T(fy);
+  c[3] = U[3 * n + idx] * T(fz);
+  c[4] = U[4 * n + idx];
+  c[5] = U[5 * n + idx] * T(fx);
+  c[6] = U[6 * n + idx] * T(fy);
+  c[7] = U[7 * n + idx] * T(fz);
+  c[8] = U[8 * n + idx];
+  cons2prim(c, q, P);
+}
+
+template <int RECON, class T>
+__device__ __forceinline__ void offsets9(const T* qm, const T* q0, const T* qp, T* dL, T* dR) {
+#pragma unroll
+  for (int v = 0; v < 9; ++v) offsets<RECON>(qm[v], q0[v], qp[v], dL[v], dR[v]);
+}
+
+// Hancock prediction of logical cell (i,j,k) (integrator_ref.jl:17-26): mh and the
+// per-axis face offsets.
+template <int RECON, class T>
+__device__ void predict_cell(const T* __restrict__ U, const Grid& G, int i, int j, int k, const StepParams<T>& P,
+                             T* mh, T (*dL)[9], T (*dR)[9]) {
+  T m0[9], qm[9], qp[9];
+  load_prim(U, G, i, j, k, m0, P);
+  load_prim(U, G, i - 1, j, k, qm, P);
+  load_prim(U, G, i + 1, j, k, qp, P);
+  offsets9<RECON>(qm, m0, qp, dL[0], dR[0]);
+  load_prim(U, G, i, j - 1, k, qm, P);
+  load_prim(U, G, i, j + 1, k, qp, P);
+  offsets9<RECON>(qm, m0, qp, dL[1], dR[1]);
+  load_prim(U, G, i, j, k - 1, qm, P);
+  load_prim(U, G, i, j, k + 1, qp, P);
+  offsets9<RECON>(qm, m0, qp, dL[2], dR[2]);
+  // hancock_edges (glm_physics.jl:225-231)
+  T acc[9], fp[9], fm[9], s[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[0][v];
+  dflux<0>(s, fp, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[0][v];
+  dflux<0>(s, fm, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[1][v];
+  dflux<1>(s, fp, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[1][v];
+  dflux<1>(s, fm, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[2][v];
+  dflux<2>(s, fp, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[2][v];
+  dflux<2>(s, fm, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+  T u0[9];
+  prim2cons(m0, u0, P);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) u0[v] -= P.hdtdx * acc[v];
+  cons2prim(u0, mh, P);
+}
+
+// Both faces of one axis for the cell: D[v] = F_lo - F_hi (integrator_ref.jl:74-125).
+template <int D, int RECON, bool CT, class T>
+__device__ void axis_faces(const T* __restrict__ U, T* __restrict__ ctp, const Grid& G, int i, int j, int k,
+                           int64_t c, const T* mhc, const T (*dLc)[9], const T (*dRc)[9], const StepParams<T>& P,
+                           T* Dd) {
+  constexpr int di = D == 0, dj = D == 1, dk = D == 2;
+  T mh[9], dL[3][9], dR[3][9];
+  T Ls[9], Rs[9], Flo[9], Fhi[9];
+  // lower face: left state from the lower neighbour's +D edge
+  predict_cell<RECON>(U, G, i - di, j - dj, k - dk, P, mh, dL, dR);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    Ls[v] = mh[v] + dR[D][v];
+    Rs[v] = mhc[v] + dLc[D][v];
+  }
+  T bl = T(0), bh = T(0);
+  int64_t cp = 0;
+  if constexpr (CT) {
+    int ii = i + di, jj = j + dj, kk = k + dk;
+    ii = ii == G.nx ? 0 : ii;
+    jj = jj == G.ny ? 0 : jj;
+    kk = kk == G.nz ? 0 : kk;
+    cp = ((int64_t)kk * G.ny + jj) * G.nx + ii;
+    bl = ctp[D * G.nc + c];
+    bh = ctp[D * G.nc + cp];
+    Ls[5 + D] = bl;
+    Rs[5 + D] = bl;
+  }
+  face_flux<D, CT>(Ls, Rs, Flo, P);
+  // upper face: right state from the upper neighbour's -D edge
+  predict_cell<RECON>(U, G, i + di, j + dj, k + dk, P, mh, dL, dR);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    Ls[v] = mhc[v] + dRc[D][v];
+    Rs[v] = mh[v] + dL[D][v];
+  }
+  if constexpr (CT) {
+    Ls[5 + D] = bh;
+    Rs[5 + D] = bh;
+  }
+  face_flux<D, CT>(Ls, Rs, Fhi, P);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) Dd[v] = Flo[v] - Fhi[v];
+  if constexpr (CT) {
+    // integrator_ref.jl:129-132: lower-face mass flux and transverse-B fluxes
+    const int64_t n = G.nc;
+    ctp[(3 + D) * n + c] = Flo[0];
+    if (D == 0) {
+      ctp[6 * n + c] = Flo[6];
+      ctp[7 * n + c] = Flo[7];
+    } else if (D == 1) {
+      ctp[8 * n + c] = Flo[5];
+      ctp[9 * n + c] = Flo[7];
+    } else {
+      ctp[10 * n + c] = Flo[5];
+      ctp[11 * n + c] = Flo[6];
+    }
+  }
+}
+
+template <int RECON, bool CT, class T>
+__global__ __launch_bounds__(256) void k_step_ref(const T* __restrict__ U, T* __restrict__ O, T* __restrict__ ctp,
+                                                  Grid G, StepParams<T> P) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G.nc) return;
+  int i = (int)(c % G.nx);
+  int j = (int)((c / G.nx) % G.ny);
+  int k = (int)(c / ((int64_t)G.nx * G.ny));
+  T mhc[9], dLc[3][9], dRc[3][9];
+  predict_cell<RECON>(U, G, i, j, k, P, mhc, dLc, dRc);
+  T Dx[9], Dy[9], Dz[9];
+  axis_faces<0, RECON, CT>(U, ctp, G, i, j, k, c, mhc, dLc, dRc, P, Dx);
+  axis_faces<1, RECON, CT>(U, ctp, G, i, j, k, c, mhc, dLc, dRc, P, Dy);
+  axis_faces<2, RECON, CT>(U, ctp, G, i, j, k, c, mhc, dLc, dRc, P, Dz);
+  const int64_t n = G.nc;
+  T U0[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) U0[v] = U[v * n + c];
+  if constexpr (CT) {
+    // integrator_ref.jl:133-138: E = -v x B with the predicted velocity and the
+    // start-of-step cell field
+    ctp[12 * n + c] = mhc[3] * U0[6] - mhc[2] * U0[7];
+    ctp[13 * n + c] = mhc[1] * U0[7] - mhc[3] * U0[5];
+    ctp[14 * n + c] = mhc[2] * U0[5] - mhc[1] * U0[6];
+  }
+  // integrator_ref.jl:140-158
+#pragma unroll
+  for (int v = 0; v < 8; ++v) O[v * n + c] = U0[v] + P.dtdx * ((Dx[v] + Dy[v]) + Dz[v]);
+  O[8 * n + c] = CT ? T(0) : (U0[8] + P.dtdx * ((Dx[8] + Dy[8]) + Dz[8])) * P.decay;
+}
+
+template <class T>
+void launch_step_ref(const T* U, T* O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct, hipStream_t st) {
+  dim3 blk(256), grd((unsigned)((G.nc + 255) / 256));
+#define PMF_REF(R, C) hipLaunchKernelGGL((k_step_ref<R, C, T>), grd, blk, 0, st, U, O, ctp, G, P)
+  if (ct) {
+    if (P.recon == RECON_PPM) PMF_REF(RECON_PPM, true);
+    else if (P.recon == RECON_PLM_MINMOD) PMF_REF(RECON_PLM_MINMOD, true);
+    else PMF_REF(RECON_PLM, true);
+  } else {
+    if (P.recon == RECON_PPM) PMF_REF(RECON_PPM, false);
+    else if (P.recon == RECON_PLM_MINMOD) PMF_REF(RECON_PLM_MINMOD, false);
+    else PMF_REF(RECON_PLM, false);
+  }
+#undef PMF_REF
+}
+
+template <bool CT, class T>
+__global__ __launch_bounds__(256) void k_riemann_batch(const T* __restrict__ L, const T* __restrict__ R,
+                                                       T* __restrict__ F, int64_t n, int dir, StepParams<T> P) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  T l[9], r[9], o[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    l[v] = L[f * 9 + v];
+    r[v] = R[f * 9 + v];
+  }
+  if (dir == 0) face_flux<0, CT>(l, r, o, P);
+  else if (dir == 1) face_flux<1, CT>(l, r, o, P);
+  else face_flux<2, CT>(l, r, o, P);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) F[f * 9 + v] = o[v];
+}
+
+template <class T>
+void launch_riemann_batch(const T* L, const T* R, T* F, int64_t n, int dir, bool ct, const StepParams<T>& P,
+                          hipStream_t st) {
+  if (n <= 0) return;
+  dim3 blk(256), grd((unsigned)((n + 255) / 256));
+  if (ct) hipLaunchKernelGGL((k_riemann_batch<true, T>), grd, blk, 0, st, L, R, F, n, dir, P);
+  else hipLaunchKernelGGL((k_riemann_batch<false, T>), grd, blk, 0, st, L, R, F, n, dir, P);
+}
+
+template void launch_step_ref<float>(const float*, float*, float*, const Grid&, const StepParams<float>&, bool,
+                                     hipStream_t);
+template void launch_step_ref<double>(const double*, double*, double*, const Grid&, const StepParams<double>&, bool,
+                                      hipStream_t);
+template void launch_riemann_batch<float>(const float*, const float*, float*, int64_t, int, bool,
+                                          const StepParams<float>&, hipStream_t);
+template void launch_riemann_batch<double>(const double*, const double*, double*, int64_t, int, bool,
+                                           const StepParams<double>&, hipStream_t);
+
+}  // namespace pmf

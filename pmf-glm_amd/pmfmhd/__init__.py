@@ -1,0 +1,361 @@
+"""pmfmhd — the MHDKernels GLM-HLLD step on MI355X, from Python.
+
+A host-side mirror of the lib/MHDKernels interface over the C ABI of libpmfmhd.so:
+``allocate_state`` (state.jl:123-141), ``step`` (driver.jl:20-37), ``evolve``
+(driver.jl:48-71), ``compute_dt`` / ``max_wavespeed`` (timestep.jl:17-41), ``max_divb`` /
+``conserved_totals`` (diagnostics.jl), the CT helpers (ct.jl) and the problem
+initialisers (problems.jl). Names, argument meaning and error text follow the reference
+(Julia's ``step!`` is ``step`` here). All compute runs in the HIP library; there is no CPU
+fallback, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import problems
+from ._lib import MHDError, check, lib, pmf_params
+from .ctinit import faces_from_cells
+
+__all__ = ["GLM_CR", "MHDState", "MHDError", "allocate_state", "step", "step_ref", "step_cube", "evolve",
+           "compute_dt", "max_wavespeed", "max_divb", "divb_norm", "conserved_totals", "max_divb_face",
+           "ct_faces_from_cells", "ct_faces_from_potential", "riemann_flux", "fields_to_host",
+           "init_alfven_wave", "init_orszag_tang", "init_brio_wu", "init_turb_field", "init_field_loop",
+           "init_config", "problems"]
+
+GLM_CR = 0.18  # driver.jl:8
+_RIEMANN = {"hlld": 0, "hll": 1, "llf": 2}
+_RECON = {"plm": 0, "ppm": 1, "plm_minmod": 2}
+_BC = {"periodic": 0, "outflow": 1, "reflecting": 2}
+_INTEG = {"ref": 0, "cube": 1}
+
+
+def _code(table, kind, value, have):
+    try:
+        return table[value]
+    except KeyError:
+        raise MHDError(f"unknown {kind} :{value} (have {have})") from None
+
+
+class MHDState:
+    """Device state: 9 SoA conserved fields, scratch set and, for CT, the face buffer."""
+
+    def __init__(self, params: pmf_params, dtype, riemann, recon, bcs, solver):
+        self._p = params
+        self.dtype = np.dtype(dtype)
+        self.riemann, self.recon, self.bcs, self._solver = riemann, recon, bcs, solver
+        h = ctypes.c_void_p()
+        check(lib().pmf_create(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().pmf_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dims(self):
+        return (self._p.nx, self._p.ny, self._p.nz)
+
+    @property
+    def ncells(self) -> int:
+        return int(np.prod(self.dims))
+
+    @property
+    def dx(self):
+        return self.dtype.type(self._p.dx)
+
+    @property
+    def gamma(self):
+        return self.dtype.type(self._p.gamma)
+
+    @property
+    def uses_ct(self) -> bool:
+        return self._solver == "ct"
+
+    @property
+    def mhd_solver(self) -> str:
+        return self._solver
+
+    def upload(self, var: int, a):
+        a = np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=self.dtype)
+        check(lib().pmf_upload(self._h, var, a.ctypes.data_as(ctypes.c_void_p), a.size))
+
+    def download(self, var: int) -> np.ndarray:
+        a = np.empty(self.ncells, dtype=self.dtype)
+        check(lib().pmf_download(self._h, var, a.ctypes.data_as(ctypes.c_void_p), a.size))
+        return a
+
+    def set_conserved(self, U):
+        """Upload a (9, ncells) conserved state (cast to the state's precision)."""
+        U = np.asarray(U)
+        if U.shape[0] != 9:
+            raise MHDError("conserved state must have 9 components")
+        for v in range(9):
+            self.upload(v, U[v])
+
+    def fields_to_host(self):
+        """state.jl:144: host copies of the 9 conserved fields."""
+        return tuple(self.download(v) for v in range(9))
+
+    def conserved(self) -> np.ndarray:
+        return np.stack(self.fields_to_host())
+
+    def ct_slot(self, q: int) -> np.ndarray:
+        """Host copy of CT slot q (0..2 = bxf, byf, bzf; state.jl:17-37 order)."""
+        a = np.empty(self.ncells, dtype=self.dtype)
+        check(lib().pmf_ct_download(self._h, q, a.ctypes.data_as(ctypes.c_void_p), a.size))
+        return a
+
+    def device_ptr(self, var: int) -> int:
+        p = ctypes.c_void_p()
+        check(lib().pmf_device_ptr(self._h, var, ctypes.byref(p)))
+        return p.value
+
+    def synchronize(self):
+        check(lib().pmf_synchronize(self._h))
+
+    def timing(self, on: bool):
+        """Bracket every step-kernel launch with HIP events on the state's stream."""
+        check(lib().pmf_timing_enable(self._h, 1 if on else 0))
+
+    def timing_read(self):
+        """(summed step-kernel milliseconds, number of timed launches) since timing(True)."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        check(lib().pmf_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def allocate_state(dims, dx, dtype=np.float32, gamma=5.0 / 3.0, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4,
+                   llf_pmin=0.0, use_hlld=True, riemann=None, bc="periodic", bcs=None, recon="plm",
+                   mhd_solver="glm") -> MHDState:
+    """state.jl:109-141. ``dtype`` is np.float32 (the bulk precision) or np.float64."""
+    if mhd_solver not in ("glm", "ct"):
+        raise MHDError(f"unknown mhd_solver :{mhd_solver} (have :glm, :ct)")
+    bcs = tuple(bcs) if bcs is not None else (bc, bc, bc)
+    if mhd_solver == "ct" and any(b != "periodic" for b in bcs):
+        raise MHDError(f"mhd_solver=:ct requires all-periodic boundaries (bcs={bcs}); "
+                       "constrained transport is periodic-only")
+    rsym = riemann if riemann is not None else ("hlld" if use_hlld else "llf")
+    p = pmf_params()
+    nx, ny, nz = (int(n) for n in dims)
+    check(lib().pmf_params_default(ctypes.byref(p), nx, ny, nz, float(dx)))
+    p.gamma, p.smallr, p.pfl = float(gamma), float(smallr), float(pfl)
+    p.llf_dmin, p.llf_pmin = float(llf_dmin), float(llf_pmin)
+    p.riemann = _code(_RIEMANN, "MHD riemann", rsym, ":hlld, :hll, :llf")
+    p.recon = _code(_RECON, "recon", recon, ":plm, :plm_minmod, :ppm")
+    for d in range(3):
+        p.bc[d] = _code(_BC, "BC", bcs[d], ":periodic, :outflow, :reflecting")
+    p.solver = 1 if mhd_solver == "ct" else 0
+    dt = np.dtype(dtype)
+    if dt not in (np.float32, np.float64):
+        raise MHDError("dtype must be float32 or float64")
+    p.precision = 1 if dt == np.float64 else 0
+    return MHDState(p, dt, rsym, recon, bcs, mhd_solver)
+
+
+def fields_to_host(s: MHDState):
+    return s.fields_to_host()
+
+
+def _integ(integrator):
+    return _code(_INTEG, "integrator", integrator, ":ref, :cube")
+
+
+def step(s: MHDState, dt, ch=None, glm_cr=GLM_CR, integrator="ref"):
+    """driver.jl:20-37 (``step!``)."""
+    code = _integ(integrator)
+    if s.uses_ct:
+        check(lib().pmf_step(s.handle, float(dt), 0.0, 0.0, code))
+        return s
+    if ch is None:
+        raise MHDError("step! needs the GLM cleaning speed `ch` (GLM states)")
+    check(lib().pmf_step(s.handle, float(dt), float(ch), float(glm_cr), code))
+    return s
+
+
+def step_ref(s: MHDState, dt, ch, decay):
+    """integrator_ref.jl:210-220 (``step_ref!``) with an explicit psi decay factor."""
+    check(lib().pmf_step_raw(s.handle, float(dt), float(ch), float(decay), 0))
+    return s
+
+
+def step_cube(s: MHDState, dt, ch, decay):
+    """integrator_cube.jl:455-463 (``step_cube!``): the fused tiled kernel."""
+    if s.uses_ct:
+        raise MHDError("step_cube!: CT is implemented on the :ref integrator only "
+                       "(mhd_solver=:ct states cannot use the :cube integrators)")
+    check(lib().pmf_step_raw(s.handle, float(dt), float(ch), float(decay), 1))
+    return s
+
+
+def max_wavespeed(s: MHDState):
+    v = ctypes.c_double()
+    check(lib().pmf_max_wavespeed(s.handle, ctypes.byref(v)))
+    return s.dtype.type(v.value)
+
+
+def compute_dt(s: MHDState, cfl=0.4):
+    """timestep.jl:43-46 -> (dt, smax) in the state's precision."""
+    dt, sm = ctypes.c_double(), ctypes.c_double()
+    check(lib().pmf_compute_dt(s.handle, float(cfl), ctypes.byref(dt), ctypes.byref(sm)))
+    return s.dtype.type(dt.value), s.dtype.type(sm.value)
+
+
+def evolve(s: MHDState, tfinal, cfl=0.4, glm_ch_fac=1.0, glm_cr=GLM_CR, integrator="ref", callback=None):
+    """driver.jl:48-71 (``evolve!``) -> (t, nsteps)."""
+    code = _integ(integrator)
+    if callback is None:
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        check(lib().pmf_evolve(s.handle, float(tfinal), float(cfl), float(glm_ch_fac), float(glm_cr), code,
+                               ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+    if not glm_ch_fac > 0:
+        raise MHDError("glm_ch_fac must be positive")
+    if not glm_cr >= 0:
+        raise MHDError("glm_cr must be non-negative")
+    T = s.dtype.type
+    t, n, tf = 0.0, 0, float(tfinal)
+    while t < tf * (1 - 1e-9):
+        dt, smax = compute_dt(s, cfl)
+        if s.uses_ct:
+            ch, dtf = T(0), float(dt)
+        else:
+            ch = T(glm_ch_fac) * T(smax)
+            dtf = float(dt) * float(smax) / float(max(T(smax), ch))
+        if t + dtf > tf:
+            dtf = tf - t
+        step(s, dtf, ch=ch, glm_cr=glm_cr, integrator=integrator)
+        t += dtf
+        n += 1
+        callback(s, t, n)
+    return t, n
+
+
+def max_divb(s: MHDState) -> float:
+    v = ctypes.c_double()
+    check(lib().pmf_max_divb(s.handle, ctypes.byref(v)))
+    return v.value
+
+
+def divb_norm(s: MHDState):
+    """Float64 (L2, Linf) of the centred div B."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    check(lib().pmf_divb_norm_fp64(s.handle, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def conserved_totals(s: MHDState) -> dict:
+    """diagnostics.jl:44-55 (float64, compensated, on the device)."""
+    out = (ctypes.c_double * 8)()
+    check(lib().pmf_conserved_totals_fp64(s.handle, out))
+    names = ("mass", "momx", "momy", "momz", "energy", "Bx", "By", "Bz")
+    return {k: out[i] for i, k in enumerate(names)}
+
+
+def max_divb_face(s: MHDState) -> float:
+    v = ctypes.c_double()
+    check(lib().pmf_max_divb_face(s.handle, ctypes.byref(v)))
+    return v.value
+
+
+def _host3(s, arrays):
+    return [np.ascontiguousarray(np.asarray(a).reshape(-1), dtype=s.dtype) for a in arrays]
+
+
+def ct_faces_from_potential(s: MHDState, Ax, Ay, Az):
+    """ct.jl:322-345."""
+    if not s.uses_ct:
+        raise MHDError("ct_faces_from_potential! needs a CT state (mhd_solver=:ct)")
+    A = _host3(s, (Ax, Ay, Az))
+    for a in A:
+        if a.size != s.ncells:
+            raise MHDError(f"vector-potential array has {a.size} entries, expected {s.ncells}")
+    check(lib().pmf_ct_faces_from_potential(s.handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in A]))
+    return s
+
+
+def ct_install_faces(s: MHDState, bx, by, bz):
+    b = _host3(s, (bx, by, bz))
+    check(lib().pmf_ct_install_faces(s.handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in b]))
+    return s
+
+
+def ct_faces_from_cells(s: MHDState) -> dict:
+    """ct.jl:246-308: divergence-free faces from the cell field (host float64 FFT)."""
+    if not s.uses_ct:
+        raise MHDError("ct_faces_from_cells! needs a CT state (mhd_solver=:ct)")
+    nx, ny, nz = s.dims
+    Bold = [s.download(5 + d).astype(np.float64).reshape(nz, ny, nx) for d in range(3)]
+    faces = faces_from_cells(*Bold)
+    ct_install_faces(s, *faces)
+    Bnew = [s.download(5 + d).astype(np.float64) for d in range(3)]
+    num = sum(np.sum((Bnew[d] - Bold[d].reshape(-1)) ** 2) for d in range(3))
+    den = sum(np.sum(Bold[d] ** 2) for d in range(3))
+    enew = sum(np.sum(Bnew[d] ** 2) for d in range(3))
+    brms = np.sqrt(enew / s.ncells)
+    return dict(cell_b_rel_l2_change=float(np.sqrt(num / den)) if den > 0 else 0.0,
+                magnetic_energy_ratio=float(enew / den) if den > 0 else 1.0,
+                max_divb_face_dx_over_brms=max_divb_face(s) * float(s.dx) / brms if brms > 0 else 0.0)
+
+
+def riemann_flux(s: MHDState, L, R, dir: int, ch=None):
+    """riemann / riemann_ct (glm_physics.jl:317-348) for n faces of (n, 9) primitive states."""
+    L = np.ascontiguousarray(L, dtype=s.dtype)
+    R = np.ascontiguousarray(R, dtype=s.dtype)
+    if L.shape != R.shape or L.ndim != 2 or L.shape[1] != 9:
+        raise MHDError("L and R must be (n, 9) arrays of the same shape")
+    F = np.empty_like(L)
+    chv = 0.0 if ch is None else float(ch)
+    check(lib().pmf_riemann_flux_batch(s.handle, L.ctypes.data_as(ctypes.c_void_p), R.ctypes.data_as(ctypes.c_void_p),
+                                       int(dir), chv, L.shape[0], F.ctypes.data_as(ctypes.c_void_p)))
+    return F
+
+
+# --- problem initialisers (problems.jl) -------------------------------------------------
+def init_alfven_wave(s: MHDState, amp=1e-3, B0=1.0, p0=0.1):
+    s.set_conserved(problems.alfven_wave(s.dims, float(s.dx), float(s.gamma), amp, B0, p0))
+    return s
+
+
+def init_orszag_tang(s: MHDState):
+    s.set_conserved(problems.orszag_tang(s.dims, float(s.dx), float(s.gamma)))
+    return s
+
+
+def init_brio_wu(s: MHDState):
+    s.set_conserved(problems.brio_wu(s.dims, float(s.gamma)))
+    return s
+
+
+def init_turb_field(s: MHDState):
+    s.set_conserved(problems.turb_field(s.dims, float(s.dx), float(s.gamma)))
+    return s
+
+
+def init_field_loop(s: MHDState, radius=0.3, amplitude=1e-3, rho0=1.0, p0=1.0, velocity=(1.0, 1.0, 0.0)):
+    """problems.jl:148-180 (for CT states the faces are the curl of an edge A_z)."""
+    s.set_conserved(problems.field_loop(s.dims, float(s.dx), float(s.gamma), radius, amplitude, rho0, p0, velocity))
+    if s.uses_ct:
+        Az = problems.field_loop_Az_edge(s.dims, float(s.dx), radius, amplitude)
+        z = np.zeros(s.ncells)
+        ct_faces_from_potential(s, z, z, Az)
+    return s
+
+
+def init_config(s: MHDState, name: str, seed: int = 1):
+    """Benchmark configuration initial condition (problems.CONFIGS) at the state's N."""
+    n = s.dims[0]
+    if not (s.dims[0] == s.dims[1] == s.dims[2]):
+        raise MHDError("benchmark configurations are cubic")
+    s.set_conserved(problems.config_state(name, seed=seed, n=n))
+    return s

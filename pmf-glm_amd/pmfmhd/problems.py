@@ -17,11 +17,22 @@ import numpy as np
 U64 = np.uint64
 
 
-def prim2cons(q, gamma):
-    """glm_physics.jl:19-23 on arrays: q is (9, ...) primitive (rho, v, p, B, psi)."""
+def prim2cons(q, gamma, out=None):
+    """glm_physics.jl:19-23 on arrays: q is a sequence of 9 primitive arrays
+    (rho, vx, vy, vz, p, Bx, By, Bz, psi); returns (9, n) float64 conserved."""
     rho, vx, vy, vz, p, bx, by, bz, psi = q
-    E = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz)
-    return np.stack([rho, rho * vx, rho * vy, rho * vz, E, bx, by, bz, psi]).astype(np.float64)
+    n = np.size(rho)
+    U = np.empty((9, n)) if out is None else out
+    U[0] = rho
+    U[1] = rho * vx
+    U[2] = rho * vy
+    U[3] = rho * vz
+    U[4] = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz) + 0.5 * (bx * bx + by * by + bz * bz)
+    U[5] = bx
+    U[6] = by
+    U[7] = bz
+    U[8] = psi
+    return U
 
 
 def _grid(dims):
@@ -117,13 +128,20 @@ def field_loop_Az_edge(dims, dx, radius=0.3, amplitude=1e-3):
 # ---------------------------------------------------------------------------
 # Counter-based random numbers (splitmix64 finaliser, pmf.jl:109-126 as the model).
 
+_C1, _C2, _C3 = U64(0x9E3779B97F4A7C15), U64(0xBF58476D1CE4E5B9), U64(0x94D049BB133111EB)
+_S30, _S27, _S31 = U64(30), U64(27), U64(31)
+
+
 def _mix64(x):
-    x = x.astype(U64)
-    with np.errstate(over="ignore"):
-        z = x + U64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> U64(30))) * U64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> U64(27))) * U64(0x94D049BB133111EB)
-    return z ^ (z >> U64(31))
+    """splitmix64 finaliser on a uint64 array (wrapping arithmetic)."""
+    with np.errstate(all="ignore"):
+        z = np.asarray(x, dtype=U64) + _C1
+        z ^= z >> _S30
+        z *= _C2
+        z ^= z >> _S27
+        z *= _C3
+        z ^= z >> _S31
+    return z
 
 
 def hash_words(seed, *keys):
@@ -156,17 +174,16 @@ def solenoidal_field(dims, seed, stream, kmin, kmax, spectral_index, rms):
     FFT and scaled to the requested rms |B|. Returns three (nz, ny, nx) float64 arrays.
     """
     nx, ny, nz = dims
-    fz = np.fft.fftfreq(nz, 1.0 / nz).astype(np.int64)
-    fy = np.fft.fftfreq(ny, 1.0 / ny).astype(np.int64)
-    fx = np.arange(nx // 2 + 1, dtype=np.int64)
-    out = []
-    # mode lists restricted to the sphere |m| <= kmax
-    MZ, MY, MX = np.meshgrid(fz, fy, fx, indexing="ij")
+    K = int(np.floor(kmax))
+    # mode lists restricted to the cube |m_d| <= kmax, then to the shell [kmin, kmax]
+    rz = np.arange(-min(K, nz // 2), min(K, (nz - 1) // 2) + 1, dtype=np.int64)
+    ry = np.arange(-min(K, ny // 2), min(K, (ny - 1) // 2) + 1, dtype=np.int64)
+    rx = np.arange(0, min(K, nx // 2) + 1, dtype=np.int64)
+    MZ, MY, MX = np.meshgrid(rz, ry, rx, indexing="ij")
     kk = np.sqrt(MX.astype(np.float64) ** 2 + MY ** 2 + MZ ** 2)
     sel = (kk >= kmin) & (kk <= kmax)
     mx, my, mz, k = MX[sel], MY[sel], MZ[sel], kk[sel]
-    # Hermitian pairing on the mx = 0 (and Nyquist) planes: key each +/- pair by the
-    # canonical member so that the coefficient of -m is the conjugate of that of m.
+    out = []
     canon = (mx > 0) | ((mx == 0) & ((my > 0) | ((my == 0) & (mz > 0))))
     if nx % 2 == 0:
         canon |= (mx == nx // 2)
@@ -199,9 +216,9 @@ def solenoidal_field(dims, seed, stream, kmin, kmax, spectral_index, rms):
 
 
 def cell_normal(dims, seed, stream):
-    """One standard normal per cell, keyed by (seed, stream, i, j, k)."""
-    i, j, k = _grid(dims)
-    return hash_normal(seed, stream, i, j, k, 0, 0)
+    """One standard normal per cell, keyed by (seed, stream, linear cell index)."""
+    c = np.arange(int(np.prod(dims)), dtype=np.int64)
+    return hash_normal(seed, stream, c)
 
 
 def random_mhd(n, seed=1, kmax=None, brms=0.3, b_index=2.0, vrms_mach=0.0, v_index=-2.0,
@@ -228,10 +245,10 @@ def random_mhd(n, seed=1, kmax=None, brms=0.3, b_index=2.0, vrms_mach=0.0, v_ind
         cs = np.sqrt(gamma * p0)
         V = solenoidal_field(dims, seed, 2, 1.0, kmax, v_index, vrms_mach * cs)
     else:
-        V = [np.zeros((n, n, n))] * 3
-    z = np.zeros(n ** 3)
-    q = np.stack([rho, V[0].reshape(-1), V[1].reshape(-1), V[2].reshape(-1), p0 * np.ones(n ** 3),
-                  B[0].reshape(-1), B[1].reshape(-1), B[2].reshape(-1), z])
+        V = [np.zeros(n ** 3)] * 3
+    p = np.full(n ** 3, p0)
+    q = [rho, V[0].reshape(-1), V[1].reshape(-1), V[2].reshape(-1), p, B[0].reshape(-1), B[1].reshape(-1),
+         B[2].reshape(-1), np.zeros(n ** 3)]
     return prim2cons(q, gamma)
 
 

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the same
+seeded inputs.
+
+Bars: float64 GPU vs float64 oracle to round-off (the kernel logic is the oracle's);
+float32 GPU vs the float32 oracle <= 1e-5 relative L2 after one step; float32 GPU vs
+the float64 oracle <= 1e-5 relative L2 after the 50-step 32^3 benchmark run (the
+measured float32 round-off floor there is ~2e-6).
+"""
+import numpy as np
+import pytest
+
+import face_np as F
+import oracle as O
+import pmfmhd as M
+from pmfmhd import problems as P
+
+pytestmark = pytest.mark.gpu
+G = 5.0 / 3.0
+
+
+def relL2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def smooth_state(dims, seed=0):
+    rng = np.random.default_rng(seed)
+    i, j, k = P._grid(dims)
+    x, y, z = [2 * np.pi * (a + 0.5) / n for a, n in zip((i, j, k), dims)]
+    ph = rng.uniform(0, 2 * np.pi, 9)
+    q = np.stack([1 + 0.2 * np.sin(x + ph[0]) * np.cos(y), 0.3 * np.sin(z + ph[1]), 0.3 * np.sin(x + ph[2]),
+                  0.3 * np.sin(y + ph[3]), 1 + 0.3 * np.cos(z + ph[4]) * np.sin(x), 0.5 + 0.2 * np.sin(y + ph[5]),
+                  0.3 * np.sin(z + ph[6]), 0.3 * np.cos(x + ph[7]), 0.05 * np.sin(x + y + ph[8])])
+    return P.prim2cons(q, G)
+
+
+def pair(dims, U, dtype=np.float64, **kw):
+    g = M.allocate_state(dims, 1.0 / dims[0], dtype=dtype, **kw)
+    g.set_conserved(U)
+    okw = {k: v for k, v in kw.items() if k != "bc"}
+    if "bc" in kw:
+        okw["bcs"] = (kw["bc"],) * 3
+    o = O.OracleState(dims, 1.0 / dims[0], dtype=dtype, **okw)
+    o.set_conserved(U)
+    return g, o
+
+
+def rand_prim(rng, n):
+    q = np.empty((n, 9))
+    q[:, 0] = rng.uniform(0.2, 2.0, n)
+    q[:, 1:4] = rng.normal(0, 0.7, (n, 3))
+    q[:, 4] = rng.uniform(0.05, 2.0, n)
+    q[:, 5:8] = rng.normal(0, 0.8, (n, 3))
+    q[:, 8] = rng.normal(0, 0.1, n)
+    return q
+
+
+@pytest.mark.parametrize("solver", ["hlld", "hll", "llf"])
+def test_riemann_flux_batch_matches_oracle(solver):
+    rng = np.random.default_rng(1)
+    n = 5000
+    L, R = rand_prim(rng, n), rand_prim(rng, n)
+    L[:40, 0] = 1e-8   # density floor
+    R[40:80, 4] = -1   # pressure floor
+    L[80:120, 0] = 5e-5  # LLF switch (llf_dmin)
+    for dtype, tol in ((np.float64, 1e-12), (np.float32, 2e-5)):
+        s = M.allocate_state((4, 4, 4), 0.25, dtype=dtype, riemann=solver)
+        for d in range(3):
+            Fg = M.riemann_flux(s, L, R, d, ch=1.7).astype(np.float64)
+            Fo = O.riemann_batch(L.astype(dtype), R.astype(dtype), d, 1.7, dtype=dtype, riemann=solver)
+            ok = np.isfinite(Fo).all(axis=1)
+            scale = np.maximum(np.abs(Fo[ok]).max(axis=0), 1.0)
+            assert np.max(np.abs(Fg[ok] - Fo[ok]) / scale) < tol, (dtype, d)
+
+
+@pytest.mark.parametrize("recon", ["plm", "ppm", "plm_minmod"])
+@pytest.mark.parametrize("solver", ["hlld", "hll", "llf"])
+def test_ref_step_float64_matches_oracle(recon, solver):
+    dims = (12, 10, 8)
+    U = smooth_state(dims, seed=3)
+    g, o = pair(dims, U, recon=recon, riemann=solver)
+    dt, smax = o.compute_dt(0.3)
+    dtg, smg = M.compute_dt(g, 0.3)
+    assert abs(smg - smax) <= 1e-14 * smax and abs(dtg - dt) <= 1e-14 * dt
+    for _ in range(2):
+        o.step(dt, ch=smax)
+        M.step(g, dt, ch=smax)
+    Ug = g.conserved()
+    for v in range(9):
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * max(np.abs(o.U[v]).max(), 1e-3), v
+
+
+@pytest.mark.parametrize("bcs", [("outflow", "periodic", "reflecting"), ("reflecting", "outflow", "periodic"),
+                                 ("outflow", "outflow", "outflow")])
+def test_ref_step_boundaries_match_oracle(bcs):
+    dims = (10, 9, 8)
+    U = smooth_state(dims, seed=5)
+    g = M.allocate_state(dims, 0.1, dtype=np.float64, bcs=bcs, recon="ppm")
+    g.set_conserved(U)
+    o = O.OracleState(dims, 0.1, bcs=bcs, recon="ppm")
+    o.set_conserved(U)
+    dt, smax = o.compute_dt(0.3)
+    for _ in range(2):
+        o.step(dt, ch=smax)
+        M.step(g, dt, ch=smax)
+    Ug = g.conserved()
+    for v in range(9):
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * max(np.abs(o.U[v]).max(), 1e-3), v
+    assert abs(M.max_divb(g) - o.max_divb()) <= 1e-12 * max(o.max_divb(), 1e-12)
+
+
+def test_config1_one_step_float32_vs_float32_oracle():
+    """North-star gate: float32 GPU vs the float32 CPU run <= 1e-5 after one step."""
+    n = 32
+    U = P.config_state("c1_32", seed=1)
+    g, o = pair((n, n, n), U, dtype=np.float32, recon="plm_minmod")
+    dt, smax = o.compute_dt(0.2)
+    o.step(float(dt), ch=smax)
+    M.step(g, float(dt), ch=smax)
+    Ug = g.conserved()
+    for v in range(9):
+        assert relL2(Ug[v], o.U[v]) <= 1e-5, v
+
+
+def test_config1_50_steps_float32_vs_float64_oracle():
+    """The 32^3 benchmark run (50 CFL steps, cfl 0.2): each run computes its own dt."""
+    n = 32
+    U = P.config_state("c1_32", seed=1)
+    g = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    g.set_conserved(U)
+    o = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
+    o.set_conserved(U)
+    for _ in range(50):
+        dt, smax = o.compute_dt(0.2)
+        o.step(float(dt), ch=smax)
+        dtg, smg = M.compute_dt(g, 0.2)
+        M.step(g, float(dtg), ch=smg)
+        assert abs(float(dtg) - float(dt)) <= 1e-5 * float(dt)
+    Ug = g.conserved()
+    errs = [relL2(Ug[v], o.U[v]) for v in range(9)]
+    assert max(errs) <= 1e-5, errs
+    tg, to = M.conserved_totals(g), o.conserved_totals()
+    assert abs(tg["mass"] - to["mass"]) <= 1e-6 * to["mass"]
+
+
+def test_evolve_alfven_float32_matches_recorded():
+    """runtests.jl:6-25 on the GPU: N=32, 48 steps, L1(By) ~ 6.919e-4 (validation-cpu.log:8)."""
+    N = 32
+    s = M.allocate_state((N, 4, 4), 1.0 / N, dtype=np.float32)
+    M.init_alfven_wave(s, amp=0.1, B0=1.0, p0=0.1)
+    t, n = M.evolve(s, 0.5, cfl=0.4)
+    By = s.download(6)[:N].astype(np.float64)
+    x = (np.arange(N) + 0.5) / N
+    L1 = np.mean(np.abs(By - P.alfven_By_exact(x, t, 1.0, amp=0.1)))
+    assert n == 48
+    assert abs(L1 - 6.919e-4) / 6.919e-4 < 5e-3
+
+
+def test_diagnostics_match_oracle():
+    dims = (16, 12, 10)
+    U = smooth_state(dims, seed=7)
+    g, o = pair(dims, U)
+    assert abs(M.max_wavespeed(g) - o.max_wavespeed()) <= 1e-14 * o.max_wavespeed()
+    assert abs(M.max_divb(g) - o.max_divb()) <= 1e-13 * o.max_divb()
+    tg, to = M.conserved_totals(g), o.conserved_totals()
+    for k in to:
+        assert abs(tg[k] - to[k]) <= 1e-13 * max(1.0, abs(to[k])), k
+    # float64 L2 norm of the centred divergence against numpy
+    l2, linf = M.divb_norm(g)
+    nx, ny, nz = dims
+    b = [U[5 + d].reshape(nz, ny, nx) for d in range(3)]
+    div = ((np.roll(b[0], -1, 2) - np.roll(b[0], 1, 2)) + (np.roll(b[1], -1, 1) - np.roll(b[1], 1, 1)) +
+           (np.roll(b[2], -1, 0) - np.roll(b[2], 1, 0))) * (dims[0] / 2.0)
+    assert abs(l2 - np.sqrt(np.mean(div ** 2))) <= 1e-12 * l2
+    assert abs(linf - np.abs(div).max()) <= 1e-12 * linf
+
+
+def test_float32_totals_are_float64_accurate():
+    n = 32
+    U = P.config_state("c1_32", seed=2)
+    g = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32)
+    g.set_conserved(U)
+    tg = M.conserved_totals(g)
+    U32 = U.astype(np.float32).astype(np.float64)
+    for v, k in enumerate(("mass", "momx", "momy", "momz", "energy", "Bx", "By", "Bz")):
+        ref = np.sum(U32[v]) / n ** 3
+        assert abs(tg[k] - ref) <= 1e-14 * max(1.0, abs(ref)), k
+
+
+# --- constrained transport ---------------------------------------------------------------
+def test_ct_float64_matches_oracle_and_fork_port():
+    from test_oracle import ct_test_hydro, ct_test_potential
+    N = 12
+    kw = dict(riemann="hll", recon="plm", mhd_solver="ct", smallr=F.FK_SMALLR, pfl=F.FK_SMALLP)
+    g = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float64, **kw)
+    o = O.OracleState((N, N, N), 1.0 / N, **kw)
+    U = ct_test_hydro(N, G)
+    A = ct_test_potential(N)
+    g.set_conserved(U)
+    o.set_conserved(U)
+    M.ct_faces_from_potential(g, *A)
+    o.ct_faces_from_potential(*A)
+    assert M.max_divb_face(g) / N < 1e-13
+    dt, _ = o.compute_dt(0.3)
+    for _ in range(5):
+        o.step(dt)
+        M.step(g, dt)
+    Ug = g.conserved()
+    for v in range(9):
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * max(np.abs(o.U[v]).max(), 1e-6), v
+    for d in range(3):
+        assert np.max(np.abs(g.ct_slot(d) - o.ct[d])) <= 1e-12 * np.abs(o.ct[d]).max()
+    assert M.max_divb_face(g) / N < 1e-12
+    assert np.all(Ug[8] == 0)
+
+
+def test_ct_float32_divergence_preserved_and_faces_from_cells():
+    N = 16
+    s = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, riemann="hlld", mhd_solver="ct")
+    M.init_turb_field(s)
+    info = M.ct_faces_from_cells(s)
+    assert info["max_divb_face_dx_over_brms"] < 1e-5
+    assert info["cell_b_rel_l2_change"] < 1e-6
+    divs = []
+    M.evolve(s, 0.1, cfl=0.3, callback=lambda st, t, n: divs.append(M.max_divb_face(st) / N))
+    assert max(divs) < 1e-5 and len(divs) > 5
+
+
+def test_errors_carry_reference_text():
+    s = M.allocate_state((8, 8, 8), 1 / 8)
+    with pytest.raises(M.MHDError, match="needs the GLM cleaning speed"):
+        M.step(s, 1e-3)
+    with pytest.raises(M.MHDError, match="unknown integrator :rk4"):
+        M.step(s, 1e-3, ch=1.0, integrator="rk4")
+    c = M.allocate_state((8, 8, 8), 1 / 8, mhd_solver="ct")
+    with pytest.raises(M.MHDError, match="CT is implemented on the :ref integrator only"):
+        M.step(c, 1e-3, integrator="cube")
+    with pytest.raises(M.MHDError, match="max_divb_face needs a CT state"):
+        M.max_divb_face(s)
