@@ -237,7 +237,8 @@ int pmf_create(const pmf_params* pp, pmf_state** out) {
         hip_check(hipMalloc(&s->ct, 21 * s->nc * s->esz), "hipMalloc(ct)");
         hip_check(hipMemsetAsync(s->ct, 0, 21 * s->nc * s->esz, s->st), "hipMemset");
       }
-      hip_check(hipMalloc(&s->red, sizeof(double) * kRedBlocks * kRedSlots), "hipMalloc(red)");
+      size_t nred = std::max<size_t>(size_t(kRedBlocks) * kRedSlots, size_t(fused_blocks(s->G)));
+      hip_check(hipMalloc(&s->red, sizeof(double) * nred), "hipMalloc(red)");
       hip_check(hipMalloc(&s->dval, sizeof(double) * 64), "hipMalloc(dval)");
       hip_check(hipHostMalloc((void**)&s->hval, sizeof(double) * 64, hipHostMallocDefault), "hipHostMalloc");
       sync(s);

@@ -21,6 +21,8 @@ template <class T>
 bool launch_step_fused(const T* U, T* O, const Grid& G, const StepParams<T>& P, double* red, double* dsmax,
                        hipStream_t st);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
+// Number of workgroups (one partial maximum each) the fused kernel launches for G.
+int64_t fused_blocks(const Grid& G);
 
 template <class T>
 void launch_wavespeed(const T* U, const Grid& G, const StepParams<T>& P, double* red, double* out, hipStream_t st);

@@ -238,3 +238,71 @@ def test_errors_carry_reference_text():
         M.step(c, 1e-3, integrator="cube")
     with pytest.raises(M.MHDError, match="max_divb_face needs a CT state"):
         M.max_divb_face(s)
+
+
+# --- the fused z-marching kernel (integrator "cube") ------------------------------------
+@pytest.mark.parametrize("dims", [(32, 16, 12), (16, 16, 5), (48, 32, 3), (16, 32, 40)])
+@pytest.mark.parametrize("recon", ["plm", "ppm", "plm_minmod"])
+def test_fused_float64_matches_oracle(dims, recon):
+    U = smooth_state(dims, seed=11)
+    g, o = pair(dims, U, recon=recon)
+    dt, smax = o.compute_dt(0.3)
+    for _ in range(2):
+        o.step(dt, ch=smax)
+        M.step(g, dt, ch=smax, integrator="cube")
+    Ug = g.conserved()
+    for v in range(9):
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * max(np.abs(o.U[v]).max(), 1e-3), (v, dims, recon)
+    # the fused epilogue's CFL maximum of the new state
+    _, sm_dev = M.compute_dt(g, 0.3)
+    assert abs(sm_dev - o.max_wavespeed()) <= 1e-14 * o.max_wavespeed()
+
+
+@pytest.mark.parametrize("solver", ["hll", "llf"])
+def test_fused_float64_other_solvers(solver):
+    dims = (32, 16, 8)
+    U = smooth_state(dims, seed=12)
+    g, o = pair(dims, U, riemann=solver)
+    dt, smax = o.compute_dt(0.3)
+    o.step(dt, ch=smax)
+    M.step(g, dt, ch=smax, integrator="cube")
+    Ug = g.conserved()
+    for v in range(9):
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * max(np.abs(o.U[v]).max(), 1e-3), v
+
+
+def test_fused_float32_config1_gates():
+    n = 32
+    U = P.config_state("c1_32", seed=1)
+    g, o = pair((n, n, n), U, dtype=np.float32, recon="plm_minmod")
+    r = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    r.set_conserved(U)
+    dt, smax = o.compute_dt(0.2)
+    o.step(float(dt), ch=smax)
+    M.step(g, float(dt), ch=smax, integrator="cube")
+    M.step(r, float(dt), ch=smax, integrator="ref")
+    Ug, Ur = g.conserved(), r.conserved()
+    for v in range(9):
+        assert relL2(Ug[v], o.U[v]) <= 1e-5, v
+        assert relL2(Ug[v], Ur[v]) <= 2e-6, v
+    # 49 more CFL steps against the float64 oracle
+    o64 = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
+    o64.set_conserved(U)
+    o64.step(float(dt), ch=smax)
+    for _ in range(49):
+        d64, s64 = o64.compute_dt(0.2)
+        o64.step(float(d64), ch=s64)
+        dg, sg = M.compute_dt(g, 0.2)
+        M.step(g, float(dg), ch=sg, integrator="cube")
+    Ug = g.conserved()
+    errs = [relL2(Ug[v], o64.U[v]) for v in range(9)]
+    assert max(errs) <= 1e-5, errs
+
+
+def test_fused_rejects_unsupported_grids():
+    s = M.allocate_state((24, 16, 8), 1 / 24)
+    with pytest.raises(M.MHDError, match="multiples of 16"):
+        M.step(s, 1e-3, ch=1.0, integrator="cube")
+    b = M.allocate_state((16, 16, 8), 1 / 16, bcs=("outflow", "periodic", "periodic"))
+    with pytest.raises(M.MHDError, match="all-periodic"):
+        M.step(b, 1e-3, ch=1.0, integrator="cube")
