@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--integrator", default="auto", choices=["auto", "cube", "ref"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
+                         "size that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the GLM step is "
@@ -107,6 +110,17 @@ def main():
     h = s.fields_to_host()
     finite = all(np.isfinite(a).all() for a in h)
     log(f"bench: {value / 1e9:.3f} Gcell/s, step kernel {avg_kernel_s * 1e3:.3f} ms, finite={finite}")
+
+    if args.calib_copy:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        src = s.device_ptr(0)
+        dst = ctypes.c_void_p()
+        M.lib().pmf_scratch_ptr(s.handle, 0, ctypes.byref(dst))
+        nbytes = 9 * cells * 4
+        rc = hip.hipMemcpy(dst, ctypes.c_void_p(src), nbytes, 3)  # hipMemcpyDeviceToDevice
+        log(f"bench: calibration copy of {nbytes} bytes rc={rc}")
 
     cpu = None
     if args.cpu_steps > 0:

@@ -138,7 +138,7 @@ void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
   bool done = false;
   if (integrator == PMF_INTEGRATOR_CUBE) {
     if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
-      throw Error("step_cube!: the fused kernel needs an all-periodic grid whose nx and ny are multiples of 16 "
+      throw Error("step_cube!: the fused kernel needs an all-periodic grid whose nx is a multiple of 16 "
                   "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
                   ")");
     mark(s);

@@ -282,8 +282,9 @@ __device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fp
 
 // riemann / riemann_ct (glm_physics.jl:317-348) on normal-frame states (modified in place:
 // floors, and the normal field replaced by the GLM b_n*, or by the CT face value).
-template <bool CT, class T>
+template <bool CT, class T, int RS = -1>
 __device__ __forceinline__ void riemann_n(T* L, T* R, T* f, const StepParams<T>& P) {
+  const int rsolve = RS >= 0 ? RS : P.rsolve;
   L[0] = dmax(L[0], P.smallr);
   L[4] = dmax(L[4], P.pfl);
   R[0] = dmax(R[0], P.smallr);
@@ -303,9 +304,9 @@ __device__ __forceinline__ void riemann_n(T* L, T* R, T* f, const StepParams<T>&
   }
   L[5] = bns;
   R[5] = bns;
-  if (P.rsolve == RS_LLF) {
+  if (rsolve == RS_LLF) {
     llf_n(L, R, ch, fbn, fpsi, f, P);
-  } else if (P.rsolve == RS_HLL) {
+  } else if (rsolve == RS_HLL) {
     hll_n(L, R, bns, fbn, fpsi, f, P);
   } else {
     bool uself = (P.llf_dmin > T(0) && dmin(L[0], R[0]) < P.llf_dmin) ||
@@ -318,13 +319,13 @@ __device__ __forceinline__ void riemann_n(T* L, T* R, T* f, const StepParams<T>&
 }
 
 // Lab-frame face flux across an axis-D face from lab-frame primitive states.
-template <int D, bool CT, class T>
+template <int D, bool CT, class T, int RS = -1>
 __device__ __forceinline__ void face_flux(const T* Lq, const T* Rq, T* F, const StepParams<T>& P) {
   constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
   T L[9] = {Lq[0], Lq[1 + N], Lq[1 + A], Lq[1 + B], Lq[4], Lq[5 + N], Lq[5 + A], Lq[5 + B], Lq[8]};
   T R[9] = {Rq[0], Rq[1 + N], Rq[1 + A], Rq[1 + B], Rq[4], Rq[5 + N], Rq[5 + A], Rq[5 + B], Rq[8]};
   T f[9];
-  riemann_n<CT>(L, R, f, P);
+  riemann_n<CT, T, RS>(L, R, f, P);
   F[0] = f[0];
   F[1 + N] = f[1];
   F[1 + A] = f[2];

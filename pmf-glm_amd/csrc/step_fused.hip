@@ -1,13 +1,14 @@
 // Fused, z-marching GLM step for periodic grids (the throughput path; the MI355X
 // counterpart of lib/MHDKernels/src/integrator_cube.jl:295-476, same unsplit scheme).
 //
-// Work decomposition. A workgroup owns a 16x16 column tile in x-y and marches it
-// through a chunk of z planes. Its 320 threads (5 wave64s) are bound for the whole
-// march to one column each: 256 owned columns plus the 64 columns of the one-cell halo
+// Work decomposition. A workgroup owns a 16x12 column tile in x-y and marches it
+// through a chunk of z planes. Its 256 threads (4 wave64s) are bound for the whole
+// march to one column each: 192 owned columns plus the 56 columns of the one-cell halo
 // ring (corners excluded), which need their Hancock prediction for the tile's boundary
-// faces. Every thread keeps the primitive states of its column for planes m-1, m, m+1
-// in registers (the z stencil); the in-plane neighbours come from one LDS plane that
-// also holds the second halo ring and the four ring corners. Per plane m:
+// faces (8 threads have no column). Two LDS planes hold the primitive states of planes
+// m and m-1 with both halo rings; the next plane enters the buffer of m-1 as soon as
+// each column's own slot there is no longer needed. With the kernel's register budget
+// a SIMD holds two waves, so two 4-wave workgroups fill a CU evenly. Per plane m:
 //   1. convert the next plane of the own column (and, for a few threads, of the extra
 //      halo cells) from conserved to primitive;
 //   2. Hancock-predict plane m in every column (cons2prim of the half-step state);
@@ -23,20 +24,26 @@
 
 namespace pmf {
 
-constexpr int FT = 16;                 // owned tile edge (x and y)
-constexpr int FPW = FT + 4;            // LDS prim plane edge (two halo rings)
-constexpr int FNP = FPW * FPW;         // 400
-constexpr int FQW = FT + 2;            // predicted region edge (one halo ring)
-constexpr int FNQ = FQW * FQW;         // 324
-constexpr int FNF = (FT + 1) * FT;     // faces per direction in a tile plane: 272
-constexpr int FTHREADS = FT * FT + 4 * FT;  // 320
+constexpr int TX = 16, TY = 12;       // owned tile (x, y)
+constexpr int NOWN = TX * TY;          // 192
+constexpr int NRING = 2 * TX + 2 * TY; // 56 (one-cell ring without corners)
+constexpr int FPW = TX + 4;            // LDS prim plane width (two halo rings)
+constexpr int FNP = FPW * (TY + 4);    // 320
+constexpr int FQW = TX + 2;            // predicted region width (one halo ring)
+constexpr int FNQ = FQW * (TY + 2);    // 252
+constexpr int FNFX = (TX + 1) * TY;    // x faces of a tile plane: 204
+constexpr int FNFY = TX * (TY + 1);    // y faces of a tile plane: 208
+constexpr int FTHREADS = 256;
 
 struct FusedGeom {
   int32_t nx, ny, nz, ntx, nty, nchunks, zc;
   int64_t plane, nc;
 };
 
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int wrapi(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
 
 template <class T>
 __device__ __forceinline__ void load_cons(const T* __restrict__ U, int64_t idx, int64_t nc, T* c) {
@@ -44,34 +51,42 @@ __device__ __forceinline__ void load_cons(const T* __restrict__ U, int64_t idx, 
   for (int v = 0; v < 9; ++v) c[v] = U[v * nc + idx];
 }
 
-// Column coordinates (tile-local) of thread t: owned 16x16, then the ring.
-__device__ __forceinline__ void column_of(int t, int& ix, int& iy) {
-  if (t < FT * FT) {
-    ix = t & (FT - 1);
-    iy = t >> 4;
-    return;
-  }
-  int h = t - FT * FT;
-  if (h < FT) { ix = h; iy = -1; }
-  else if (h < 2 * FT) { ix = h - FT; iy = FT; }
-  else if (h < 3 * FT) { ix = -1; iy = h - 2 * FT; }
-  else { ix = FT; iy = h - 3 * FT; }
+// Ring cell h (0..NRING-1) at distance r from the tile: bottom row, top row, left
+// column, right column.
+__device__ __forceinline__ void ring_cell(int h, int r, int& ix, int& iy) {
+  if (h < TX) { ix = h; iy = -r; }
+  else if (h < 2 * TX) { ix = h - TX; iy = TY - 1 + r; }
+  else if (h < 2 * TX + TY) { ix = -r; iy = h - 2 * TX; }
+  else { ix = TX - 1 + r; iy = h - 2 * TX - TY; }
 }
 
-// Extra cells of the LDS plane: the second halo ring (threads 256..319) and the four
-// ring corners (threads 0..3).
-__device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
-  if (t >= FT * FT) {
-    int h = t - FT * FT;
-    if (h < FT) { ix = h; iy = -2; }
-    else if (h < 2 * FT) { ix = h - FT; iy = FT + 1; }
-    else if (h < 3 * FT) { ix = -2; iy = h - 2 * FT; }
-    else { ix = FT + 1; iy = h - 3 * FT; }
+// Column of thread t: owned 16x12, then the ring; threads past the ring have none.
+__device__ __forceinline__ bool column_of(int t, int& ix, int& iy) {
+  if (t < NOWN) {
+    ix = t % TX;
+    iy = t / TX;
     return true;
   }
-  if (t < 4) {
-    ix = (t & 1) ? FT : -1;
-    iy = (t & 2) ? FT : -1;
+  if (t < NOWN + NRING) {
+    ring_cell(t - NOWN, 1, ix, iy);
+    return true;
+  }
+  ix = 0;
+  iy = 0;
+  return false;
+}
+
+// Extra cells of the LDS plane: the second halo ring (threads NOWN..NOWN+55) and the
+// four corners of the first ring (the next four threads).
+__device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
+  if (t >= NOWN && t < NOWN + NRING) {
+    ring_cell(t - NOWN, 2, ix, iy);
+    return true;
+  }
+  int c = t - NOWN - NRING;
+  if (c >= 0 && c < 4) {
+    ix = (c & 1) ? TX : -1;
+    iy = (c & 2) ? TY : -1;
     return true;
   }
   return false;
@@ -80,13 +95,13 @@ __device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
 __device__ __forceinline__ int pidx(int ix, int iy) { return (iy + 2) * FPW + (ix + 2); }
 __device__ __forceinline__ int qidx(int ix, int iy) { return (iy + 1) * FQW + (ix + 1); }
 
-template <class T, int RECON>
+template <class T, int RECON, int RS>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U, T* __restrict__ O, FusedGeom g,
                                                          StepParams<T> P, double* __restrict__ red) {
-  __shared__ T sP[9 * FNP];       // primitive plane m (two halo rings)
+  __shared__ T sP[2][9 * FNP];    // primitive planes m (current) and m-1, alternating
   __shared__ T sLX[9 * FNQ];      // +x face states, then x-face fluxes
   __shared__ T sLY[9 * FNQ];      // +y face states, then y-face fluxes
-  __shared__ double sMax[FTHREADS / 64];
+  __shared__ T sMax[FTHREADS / 64];
 
   // XCD-aware tile order: dispatch round-robins workgroups over the 8 XCDs, so
   // logical tile (w % 8) * (nwg / 8) + w / 8 gives each XCD a contiguous tile range.
@@ -96,104 +111,117 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
-  const int x0 = tx * FT, y0 = ty * FT;
+  const int x0 = tx * TX, y0 = ty * TY;
   const int z0 = chunk * g.zc;
   const int zc = min(g.zc, g.nz - z0);
 
   const int t = threadIdx.x;
   int ix, iy;
-  column_of(t, ix, iy);
-  const bool owned = t < FT * FT;
+  const bool has_col = column_of(t, ix, iy);
+  const bool owned = t < NOWN;
+  const bool stores = owned && y0 + iy < g.ny;  // ragged last tile row: compute, do not store
   const int64_t col = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
   int ex = 0, ey = 0;
   const bool has_extra = extra_of(t, ex, ey);
   const int64_t ecol = has_extra ? (int64_t)wrapi(y0 + ey, g.ny) * g.nx + wrapi(x0 + ex, g.nx) : 0;
-  const bool xsolver = iy >= 0 && iy < FT && ix >= 0;   // owned + right ring column
-  const bool ysolver = ix >= 0 && ix < FT && iy >= 0;   // owned + top ring row
+  const bool xsolver = has_col && iy >= 0 && iy < TY && ix >= 0;  // owned + right ring column
+  const bool ysolver = has_col && ix >= 0 && ix < TX && iy >= 0;  // owned + top ring row
+  const int me = pidx(ix, iy);
 
-  T Pa[9], Pb[9], Pc[9], Xc[9];
-  T Lzp[9], Fzp[9], Dxyp[9];
+  // Column registers: Lzp = left state of the z face below the current plane, Fzp = flux
+  // of the z face below the plane being updated, Dxy = that plane's x/y divergence.
+  // The LDS buffer of plane m-1 is refilled with plane m+1 while plane m is processed:
+  // each thread overwrites its own column slot (read only by itself, for the z slope)
+  // after its prediction, and the extra cells' slots are never read during plane m.
+  T Lzp[9], Fzp[9], Dxy[9];
   {
-    T c[9];
-    load_cons(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, g.nc, c);
-    cons2prim(c, Pa, P);
-    load_cons(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, g.nc, c);
-    cons2prim(c, Pb, P);
+    T c[9], q[9];
+    if (has_col) {
+      load_cons(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, g.nc, c);
+      cons2prim(c, q, P);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) sP[(z0 - 2) & 1][v * FNP + me] = q[v];
+      load_cons(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, g.nc, c);
+      cons2prim(c, q, P);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) sP[(z0 - 1) & 1][v * FNP + me] = q[v];
+    }
   }
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     Lzp[v] = T(0);
     Fzp[v] = T(0);
-    Dxyp[v] = T(0);
+    Dxy[v] = T(0);
   }
-  double smax = 0.0;
+  T smax = T(0);
 
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
-    // (1) own column of plane m+1, extra cells of plane m (converted last iteration)
-    {
-      T c[9];
-      load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, g.nc, c);
-      cons2prim(c, Pc, P);
-    }
-    T Ucur[9];
-    const bool upd = owned && m - 1 >= z0;
-    if (upd) load_cons(U, (int64_t)(m - 1) * g.plane + col, g.nc, Ucur);
-#pragma unroll
-    for (int v = 0; v < 9; ++v) sP[v * FNP + pidx(ix, iy)] = Pb[v];
-    if (has_extra && ring_work) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) sP[v * FNP + pidx(ex, ey)] = Xc[v];
-    }
+    const T* __restrict__ cur = sP[m & 1];
+    T* __restrict__ nxt = sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
+    // (1) plane m+1 of the own column and of the extra cell (in flight during the barrier)
+    T Pn[9], Xc[9];
+    if (has_col) load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, g.nc, Pn);
+    const bool extra_now = has_extra && m + 1 < z0 + zc;
+    if (extra_now) load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, g.nc, Xc);
     __syncthreads();
-    if (has_extra && m + 1 < z0 + zc) {
-      T c[9];
-      load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, g.nc, c);
-      cons2prim(c, Xc, P);
+    if (extra_now) {
+      T q[9];
+      cons2prim(Xc, q, P);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) nxt[v * FNP + pidx(ex, ey)] = q[v];
+    }
+    if (has_col) {
+      T q[9];
+      cons2prim(Pn, q, P);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) Pn[v] = q[v];
     }
     // (2) Hancock prediction of plane m (glm_physics.jl:225-231, integrator_ref.jl:17-26)
-    T Rx[9], Ry[9], Rz[9], Lzn[9];
-    if (owned || ring_work) {
-      T dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
+    T Rx[9], Ry[9], Rz[9];
+    if (owned || (has_col && ring_work)) {
+      T m0[9], dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
-        offsets<RECON>(sP[v * FNP + pidx(ix - 1, iy)], Pb[v], sP[v * FNP + pidx(ix + 1, iy)], dLx[v], dRx[v]);
-        offsets<RECON>(sP[v * FNP + pidx(ix, iy - 1)], Pb[v], sP[v * FNP + pidx(ix, iy + 1)], dLy[v], dRy[v]);
-        offsets<RECON>(Pa[v], Pb[v], Pc[v], dLz[v], dRz[v]);
+        m0[v] = cur[v * FNP + me];
+        offsets<RECON>(cur[v * FNP + me - 1], m0[v], cur[v * FNP + me + 1], dLx[v], dRx[v]);
+        offsets<RECON>(cur[v * FNP + me - FPW], m0[v], cur[v * FNP + me + FPW], dLy[v], dRy[v]);
+        offsets<RECON>(nxt[v * FNP + me], m0[v], Pn[v], dLz[v], dRz[v]);
+        nxt[v * FNP + me] = Pn[v];
       }
       T acc[9], fp[9], fm[9], s[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dRx[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRx[v];
       dflux<0>(s, fp, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dLx[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLx[v];
       dflux<0>(s, fm, P.igm1);
 #pragma unroll
       for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dRy[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRy[v];
       dflux<1>(s, fp, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dLy[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLy[v];
       dflux<1>(s, fm, P.igm1);
 #pragma unroll
       for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dRz[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRz[v];
       dflux<2>(s, fp, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = Pb[v] + dLz[v];
+      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLz[v];
       dflux<2>(s, fm, P.igm1);
 #pragma unroll
       for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
       T uh[9], mh[9];
-      prim2cons(Pb, uh, P);
+      prim2cons(m0, uh, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) uh[v] -= P.hdtdx * acc[v];
       cons2prim(uh, mh, P);
-      // face states (integrator_ref.jl:27,74-82): +d edge feeds the upper face as its
-      // left state, -d edge the lower face as its right state
+      // face states (integrator_ref.jl:27,74-82): the +d edge is the left state of the
+      // upper face, the -d edge the right state of the lower face
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
         sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
@@ -201,65 +229,69 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U
         Rx[v] = mh[v] + dLx[v];
         Ry[v] = mh[v] + dLy[v];
         Rz[v] = mh[v] + dLz[v];
-        Lzn[v] = mh[v] + dRz[v];
+      }
+      // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
+      // whose inputs all exist now (integrator_ref.jl:140-158)
+      if (owned) {
+        T Fz[9];
+        if (!first) face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) Lzp[v] = mh[v] + dRz[v];
+        if (m - 1 >= z0 && stores) {
+          T o[9];
+          const int64_t idx = (int64_t)(m - 1) * g.plane + col;
+          load_cons(U, idx, g.nc, o);
+#pragma unroll
+          for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+          o[8] *= P.decay;
+#pragma unroll
+          for (int v = 0; v < 9; ++v) O[v * g.nc + idx] = o[v];
+          smax = nanmax(smax, cell_wavespeed(o, P));
+        }
+        if (!first) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) Fzp[v] = Fz[v];
+        }
       }
     }
+    else if (has_col) {
+      // ring column with no prediction on this plane: still deposit plane m+1
+#pragma unroll
+      for (int v = 0; v < 9; ++v) nxt[v * FNP + me] = Pn[v];
+    }
     __syncthreads();
-    // (3) faces: x and y faces of plane m, z face m-1/2
+    // (3b) x and y faces of plane m, each solved once
     const bool xy = !(first || last);
-    T Fx[9], Fy[9], Fz[9];
-    if (xy && xsolver) {
-      T L[9];
+    if (xy) {
+      T Fx[9], Fy[9];
+      if (xsolver) {
+        T L[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) L[v] = sLX[v * FNQ + qidx(ix - 1, iy)];
-      face_flux<0, false>(L, Rx, Fx, P);
-    }
-    if (xy && ysolver) {
-      T L[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
-      face_flux<1, false>(L, Ry, Fy, P);
-    }
-    if (owned && !first) face_flux<2, false>(Lzp, Rz, Fz, P);
-    __syncthreads();
-    if (xy && xsolver) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) sLX[v * FNF + iy * (FT + 1) + ix] = Fx[v];
-    }
-    if (xy && ysolver) {
-#pragma unroll
-      for (int v = 0; v < 9; ++v) sLY[v * FNF + iy * FT + ix] = Fy[v];
-    }
-    __syncthreads();
-    if (owned) {
-      // (4) conservative update of plane m-1 (integrator_ref.jl:140-158)
-      if (upd) {
-        T o[9];
-#pragma unroll
-        for (int v = 0; v < 9; ++v) o[v] = Ucur[v] + P.dtdx * (Dxyp[v] + (Fzp[v] - Fz[v]));
-        o[8] *= P.decay;
-        const int64_t idx = (int64_t)(m - 1) * g.plane + col;
-#pragma unroll
-        for (int v = 0; v < 9; ++v) O[v * g.nc + idx] = o[v];
-        smax = nanmax(smax, (double)cell_wavespeed(o, P));
+        for (int v = 0; v < 9; ++v) L[v] = sLX[v * FNQ + qidx(ix - 1, iy)];
+        face_flux<0, false, T, RS>(L, Rx, Fx, P);
       }
-      if (xy) {
+      if (ysolver) {
+        T L[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
+        face_flux<1, false, T, RS>(L, Ry, Fy, P);
+      }
+      __syncthreads();
+      if (xsolver) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
+      }
+      if (ysolver) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) sLY[v * FNFY + iy * TX + ix] = Fy[v];
+      }
+      __syncthreads();
+      if (owned) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
-          Dxyp[v] = (sLX[v * FNF + iy * (FT + 1) + ix] - sLX[v * FNF + iy * (FT + 1) + ix + 1]) +
-                    (sLY[v * FNF + iy * FT + ix] - sLY[v * FNF + (iy + 1) * FT + ix]);
+          Dxy[v] = (sLX[v * FNFX + iy * (TX + 1) + ix] - sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
+                   (sLY[v * FNFY + iy * TX + ix] - sLY[v * FNFY + (iy + 1) * TX + ix]);
       }
-      if (!first) {
-#pragma unroll
-        for (int v = 0; v < 9; ++v) Fzp[v] = Fz[v];
-      }
-#pragma unroll
-      for (int v = 0; v < 9; ++v) Lzp[v] = Lzn[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      Pa[v] = Pb[v];
-      Pb[v] = Pc[v];
     }
   }
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
@@ -268,9 +300,9 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U
   if ((t & 63) == 0) sMax[t >> 6] = smax;
   __syncthreads();
   if (t == 0) {
-    double r = sMax[0];
+    T r = sMax[0];
     for (int i = 1; i < FTHREADS / 64; ++i) r = nanmax(r, sMax[i]);
-    red[blockIdx.x] = r;
+    red[blockIdx.x] = (double)r;
   }
 }
 
@@ -290,8 +322,8 @@ static FusedGeom fused_geom(const Grid& G, int zc) {
   g.nx = G.nx;
   g.ny = G.ny;
   g.nz = G.nz;
-  g.ntx = G.nx / FT;
-  g.nty = G.ny / FT;
+  g.ntx = G.nx / TX;
+  g.nty = (G.ny + TY - 1) / TY;
   g.zc = zc;
   g.nchunks = (G.nz + zc - 1) / zc;
   g.plane = (int64_t)G.nx * G.ny;
@@ -303,21 +335,21 @@ bool fused_supported(const Grid& G, int precision, int recon, int rsolve) {
   (void)precision;
   (void)recon;
   (void)rsolve;
-  return G.bc[0] == BC_PERIODIC && G.bc[1] == BC_PERIODIC && G.bc[2] == BC_PERIODIC && G.nx % FT == 0 &&
-         G.ny % FT == 0 && G.nz >= 2;
+  return G.bc[0] == BC_PERIODIC && G.bc[1] == BC_PERIODIC && G.bc[2] == BC_PERIODIC && G.nx % TX == 0 &&
+         G.ny >= 2 && G.nz >= 2;
 }
 
 // z-chunk length: long enough that the two extra prediction planes per chunk stay
 // small, short enough to give a few workgroups per CU.
 static int pick_zc(const Grid& G) {
-  int64_t tiles = (int64_t)(G.nx / FT) * (G.ny / FT);
+  int64_t tiles = (int64_t)(G.nx / TX) * ((G.ny + TY - 1) / TY);
   int zc = 32;
   while (zc > 4 && tiles * ((G.nz + zc - 1) / zc) < 2048) zc >>= 1;
   return zc < G.nz ? zc : G.nz;
 }
 
 int64_t fused_blocks(const Grid& G) {
-  if (G.nx % FT || G.ny % FT) return 0;
+  if (G.nx % TX) return 0;
   FusedGeom g = fused_geom(G, pick_zc(G));
   return (int64_t)g.ntx * g.nty * g.nchunks;
 }
@@ -329,12 +361,20 @@ bool launch_step_fused(const T* U, T* O, const Grid& G, const StepParams<T>& P, 
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
-  if (P.recon == RECON_PPM)
-    hipLaunchKernelGGL((k_step_fused<T, RECON_PPM>), grd, blk, 0, st, U, O, g, P, red);
-  else if (P.recon == RECON_PLM_MINMOD)
-    hipLaunchKernelGGL((k_step_fused<T, RECON_PLM_MINMOD>), grd, blk, 0, st, U, O, g, P, red);
-  else
-    hipLaunchKernelGGL((k_step_fused<T, RECON_PLM>), grd, blk, 0, st, U, O, g, P, red);
+#define PMF_FUSED(R, S) hipLaunchKernelGGL((k_step_fused<T, R, S>), grd, blk, 0, st, U, O, g, P, red)
+#define PMF_FUSED_RS(R)                                  \
+  if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
+  else if (P.rsolve == RS_LLF) PMF_FUSED(R, RS_LLF);     \
+  else PMF_FUSED(R, RS_HLLD);
+  if (P.recon == RECON_PPM) {
+    PMF_FUSED_RS(RECON_PPM)
+  } else if (P.recon == RECON_PLM_MINMOD) {
+    PMF_FUSED_RS(RECON_PLM_MINMOD)
+  } else {
+    PMF_FUSED_RS(RECON_PLM)
+  }
+#undef PMF_FUSED_RS
+#undef PMF_FUSED
   hipLaunchKernelGGL(k_fold_max_flat, dim3(1), dim3(256), 0, st, red, (int)nb, dsmax);
   return true;
 }

@@ -301,7 +301,7 @@ def test_fused_float32_config1_gates():
 
 def test_fused_rejects_unsupported_grids():
     s = M.allocate_state((24, 16, 8), 1 / 24)
-    with pytest.raises(M.MHDError, match="multiples of 16"):
+    with pytest.raises(M.MHDError, match="multiple of 16"):
         M.step(s, 1e-3, ch=1.0, integrator="cube")
     b = M.allocate_state((16, 16, 8), 1 / 16, bcs=("outflow", "periodic", "periodic"))
     with pytest.raises(M.MHDError, match="all-periodic"):
