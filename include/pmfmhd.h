@@ -51,6 +51,10 @@ int pmf_params_default(pmf_params* p, int32_t nx, int32_t ny, int32_t nz, double
  * face/workspace buffer (state.jl:44-58), on the current HIP device. */
 int pmf_create(const pmf_params* p, pmf_state** out);
 int pmf_destroy(pmf_state* s);
+/* A state without fields, for a caller that owns its arrays (the reference's MHDState
+ * holds nine separate device vectors, state.jl:63-65): it carries the parameters, the
+ * stream and the reduction workspace used by pmf_step_into / pmf_max_wavespeed_of. */
+int pmf_create_workspace(const pmf_params* p, pmf_state** out);
 
 /* Host <-> device copies of one conserved variable (n = number of cells; the host
  * buffer holds elements of the state's precision). fields_to_host: state.jl:144. */
@@ -84,6 +88,15 @@ int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax);
 int pmf_step_raw(pmf_state* s, double dt, double ch, double decay, int32_t integrator);
 /* step! (driver.jl:20-37): decay = exp(-glm_cr*ch*dt/dx) in the state's precision. */
 int pmf_step(pmf_state* s, double dt, double ch, double glm_cr, int32_t integrator);
+/* _cube_launch! / _launch_step_ref! seam (integrator_cube.jl:467-476,
+ * integrator_ref.jl:189-201): one step from the caller's nine input field arrays U[0..8]
+ * into its nine output arrays O[0..8] (no swap; the caller swaps as step_ref! does), then
+ * for CT states ct_update! on O using the caller's 21-slot packed buffer ct (NULL for
+ * GLM). Asynchronous on the state's stream. */
+int pmf_step_into(pmf_state* s, const void* const* U, void* const* O, void* ct, double dt, double ch,
+                  double decay, int32_t integrator);
+/* max_wavespeed! (timestep.jl:17-23) of the caller's nine field arrays. */
+int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax);
 /* evolve! (driver.jl:48-71) without a callback: CFL steps until t >= tfinal(1-1e-9),
  * ch = glm_ch_fac*smax, dt scaled by smax/max(smax,ch), the last step clipped. */
 int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, double glm_cr,

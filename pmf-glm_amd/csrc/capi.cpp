@@ -72,6 +72,38 @@ void check_state(const pmf_state* s) {
   if (!s) throw Error("null pmf_state");
 }
 
+void check_fields(const pmf_state* s) {
+  check_state(s);
+  if (!s->U) throw Error("this pmf_state is a workspace without fields; use the *_into / *_of entry points");
+}
+
+template <class T> Fields<T> fields_at(void* base, int64_t nc) {
+  Fields<T> f;
+  for (int v = 0; v < 9; ++v) f.v[v] = (T*)base + v * nc;
+  return f;
+}
+template <class T> Fields<const T> cfields_at(const void* base, int64_t nc) {
+  Fields<const T> f;
+  for (int v = 0; v < 9; ++v) f.v[v] = (const T*)base + v * nc;
+  return f;
+}
+template <class T> Fields<T> fields_from(void* const* ptrs) {
+  Fields<T> f;
+  for (int v = 0; v < 9; ++v) {
+    if (!ptrs[v]) throw Error("null field pointer");
+    f.v[v] = (T*)ptrs[v];
+  }
+  return f;
+}
+template <class T> Fields<const T> cfields_from(const void* const* ptrs) {
+  Fields<const T> f;
+  for (int v = 0; v < 9; ++v) {
+    if (!ptrs[v]) throw Error("null field pointer");
+    f.v[v] = (const T*)ptrs[v];
+  }
+  return f;
+}
+
 template <class T> StepParams<T> make_params(const pmf_state* s, double dt, double ch, double decay) {
   const pmf_params& p = s->p;
   StepParams<T> P;
@@ -115,9 +147,11 @@ void fetch(pmf_state* s, int first, int count) {
 double smax_device(pmf_state* s) {
   if (!s->smax_valid) {
     if (s->p.precision == PMF_F64)
-      launch_wavespeed<double>((const double*)s->U, s->G, make_params<double>(s, 0, 0, 1), s->red, s->dval, s->st);
+      launch_wavespeed<double>(cfields_at<double>(s->U, s->nc), s->G, make_params<double>(s, 0, 0, 1), s->red,
+                               s->dval, s->st);
     else
-      launch_wavespeed<float>((const float*)s->U, s->G, make_params<float>(s, 0, 0, 1), s->red, s->dval, s->st);
+      launch_wavespeed<float>(cfields_at<float>(s->U, s->nc), s->G, make_params<float>(s, 0, 0, 1), s->red, s->dval,
+                              s->st);
     hip_check(hipGetLastError(), "wavespeed launch");
     s->smax_valid = true;
   }
@@ -125,9 +159,11 @@ double smax_device(pmf_state* s) {
   return s->hval[0];
 }
 
-// step_ref! / step_cube! with explicit (dt, ch, decay): kernel into scratch, swap, then
-// ct_update! for CT states (integrator_ref.jl:210-220).
-void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
+// One step from fields `in` into fields `out` (integrator_ref.jl:69-172 / the cube launch),
+// then ct_update! on `out` for CT states (ct.jl:158-173). No swap.
+template <class T>
+void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double dt, double ch, double decay,
+                 int integrator) {
   const bool ct = s->p.solver == PMF_SOLVER_CT;
   if (integrator != PMF_INTEGRATOR_REF && integrator != PMF_INTEGRATOR_CUBE)
     throw Error("unknown integrator :" + std::to_string(integrator) + " (have :ref, :cube)");
@@ -135,43 +171,40 @@ void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
     throw Error("step!: CT is implemented on the :ref integrator only (mhd_solver=:ct states cannot use the :cube "
                 "integrators)");
   if (!ct && !(ch > 0.0)) throw Error("step! needs the GLM cleaning speed `ch` (GLM states)");
-  bool done = false;
+  if (ct && !ctp) throw Error("ct_update! needs a CT state");
   if (integrator == PMF_INTEGRATOR_CUBE) {
     if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
       throw Error("step_cube!: the fused kernel needs an all-periodic grid whose nx is a multiple of 16 "
                   "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
                   ")");
     mark(s);
-    if (s->p.precision == PMF_F64)
-      done = launch_step_fused<double>((const double*)s->U, (double*)s->S, s->G,
-                                       make_params<double>(s, dt, ch, decay), s->red, s->dval, s->st);
-    else
-      done = launch_step_fused<float>((const float*)s->U, (float*)s->S, s->G, make_params<float>(s, dt, ch, decay),
-                                      s->red, s->dval, s->st);
+    bool done = launch_step_fused<T>(in, out, s->G, make_params<T>(s, dt, ch, decay), s->red, s->dval, s->st);
     mark(s);
     if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
-    s->smax_valid = true;  // the fused epilogue reduced the new state's CFL speed
   } else {
     mark(s);
-    if (s->p.precision == PMF_F64)
-      launch_step_ref<double>((const double*)s->U, (double*)s->S, (double*)s->ct, s->G,
-                              make_params<double>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
-    else
-      launch_step_ref<float>((const float*)s->U, (float*)s->S, (float*)s->ct, s->G,
-                             make_params<float>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
+    launch_step_ref<T>(in, out, ctp, s->G, make_params<T>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
     mark(s);
-    s->smax_valid = false;
   }
   hip_check(hipGetLastError(), "step launch");
-  std::swap(s->U, s->S);
   if (ct) {
-    if (s->p.precision == PMF_F64)
-      launch_ct_update<double>((double*)s->U, (double*)s->ct, s->G, double(dt) / s->p.dx, false, s->st);
-    else
-      launch_ct_update<float>((float*)s->U, (float*)s->ct, s->G, float(dt) / float(s->p.dx), false, s->st);
+    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), false, s->st);
     hip_check(hipGetLastError(), "ct_update launch");
-    s->smax_valid = false;
   }
+}
+
+// step_ref! / step_cube! on the state's own fields: into scratch, then swap.
+void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
+  check_fields(s);
+  if (s->p.precision == PMF_F64)
+    step_fields<double>(s, cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc), (double*)s->ct, dt, ch,
+                        decay, integrator);
+  else
+    step_fields<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), (float*)s->ct, dt, ch,
+                       decay, integrator);
+  std::swap(s->U, s->S);
+  // the fused epilogue reduced the new state's CFL speed into dval[0]
+  s->smax_valid = integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM;
 }
 
 // decay = exp(-T(glm_cr)*T(ch)*T(dt)/dx) in the state's precision (driver.jl:29).
@@ -205,29 +238,30 @@ int pmf_params_default(pmf_params* p, int32_t nx, int32_t ny, int32_t nz, double
   });
 }
 
-int pmf_create(const pmf_params* pp, pmf_state** out) {
-  return guard([&] {
-    if (!pp || !out) throw Error("null argument");
-    const pmf_params& p = *pp;
-    if (p.nx < 1 || p.ny < 1 || p.nz < 1) throw Error("grid dimensions must be positive");
-    if (!(p.dx > 0)) throw Error("dx must be positive");
-    if (p.riemann < 0 || p.riemann > 2) throw Error("unknown MHD riemann code (have :hlld, :hll, :llf)");
-    if (p.recon < 0 || p.recon > 2) throw Error("unknown recon code (have :plm, :plm_minmod, :ppm)");
-    for (int d = 0; d < 3; ++d)
-      if (p.bc[d] < 0 || p.bc[d] > 2) throw Error("unknown BC code (have :periodic, :outflow, :reflecting)");
-    if (p.solver != PMF_SOLVER_GLM && p.solver != PMF_SOLVER_CT) throw Error("unknown mhd_solver (have :glm, :ct)");
-    if (p.precision != PMF_F32 && p.precision != PMF_F64) throw Error("precision must be PMF_F32 or PMF_F64");
-    if (p.solver == PMF_SOLVER_CT && (p.bc[0] || p.bc[1] || p.bc[2]))
-      throw Error("mhd_solver=:ct requires all-periodic boundaries; constrained transport is periodic-only");
-    auto* s = new pmf_state();
-    try {
-      s->p = p;
-      s->nc = int64_t(p.nx) * p.ny * p.nz;
-      s->G = Grid{p.nx, p.ny, p.nz, {p.bc[0], p.bc[1], p.bc[2]}, s->nc};
-      s->esz = p.precision == PMF_F64 ? 8 : 4;
-      hip_check(hipGetDevice(&s->device), "hipGetDevice");
-      hip_check(hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking), "hipStreamCreate");
-      s->st = s->own;
+namespace {
+void create_state(const pmf_params* pp, pmf_state** out, bool with_fields) {
+  if (!pp || !out) throw Error("null argument");
+  const pmf_params& p = *pp;
+  if (p.nx < 1 || p.ny < 1 || p.nz < 1) throw Error("grid dimensions must be positive");
+  if (!(p.dx > 0)) throw Error("dx must be positive");
+  if (p.riemann < 0 || p.riemann > 2) throw Error("unknown MHD riemann code (have :hlld, :hll, :llf)");
+  if (p.recon < 0 || p.recon > 2) throw Error("unknown recon code (have :plm, :plm_minmod, :ppm)");
+  for (int d = 0; d < 3; ++d)
+    if (p.bc[d] < 0 || p.bc[d] > 2) throw Error("unknown BC code (have :periodic, :outflow, :reflecting)");
+  if (p.solver != PMF_SOLVER_GLM && p.solver != PMF_SOLVER_CT) throw Error("unknown mhd_solver (have :glm, :ct)");
+  if (p.precision != PMF_F32 && p.precision != PMF_F64) throw Error("precision must be PMF_F32 or PMF_F64");
+  if (p.solver == PMF_SOLVER_CT && (p.bc[0] || p.bc[1] || p.bc[2]))
+    throw Error("mhd_solver=:ct requires all-periodic boundaries; constrained transport is periodic-only");
+  auto* s = new pmf_state();
+  try {
+    s->p = p;
+    s->nc = int64_t(p.nx) * p.ny * p.nz;
+    s->G = Grid{p.nx, p.ny, p.nz, {p.bc[0], p.bc[1], p.bc[2]}, s->nc};
+    s->esz = p.precision == PMF_F64 ? 8 : 4;
+    hip_check(hipGetDevice(&s->device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking), "hipStreamCreate");
+    s->st = s->own;
+    if (with_fields) {
       size_t bytes = 9 * s->nc * s->esz;
       hip_check(hipMalloc(&s->U, bytes), "hipMalloc(U)");
       hip_check(hipMalloc(&s->S, bytes), "hipMalloc(scratch)");
@@ -237,17 +271,26 @@ int pmf_create(const pmf_params* pp, pmf_state** out) {
         hip_check(hipMalloc(&s->ct, 21 * s->nc * s->esz), "hipMalloc(ct)");
         hip_check(hipMemsetAsync(s->ct, 0, 21 * s->nc * s->esz, s->st), "hipMemset");
       }
-      size_t nred = std::max<size_t>(size_t(kRedBlocks) * kRedSlots, size_t(fused_blocks(s->G)));
-      hip_check(hipMalloc(&s->red, sizeof(double) * nred), "hipMalloc(red)");
-      hip_check(hipMalloc(&s->dval, sizeof(double) * 64), "hipMalloc(dval)");
-      hip_check(hipHostMalloc((void**)&s->hval, sizeof(double) * 64, hipHostMallocDefault), "hipHostMalloc");
-      sync(s);
-    } catch (...) {
-      pmf_destroy(s);
-      throw;
     }
-    *out = s;
-  });
+    size_t nred = std::max<size_t>(size_t(kRedBlocks) * kRedSlots, size_t(fused_blocks(s->G)));
+    hip_check(hipMalloc(&s->red, sizeof(double) * nred), "hipMalloc(red)");
+    hip_check(hipMalloc(&s->dval, sizeof(double) * 64), "hipMalloc(dval)");
+    hip_check(hipHostMalloc((void**)&s->hval, sizeof(double) * 64, hipHostMallocDefault), "hipHostMalloc");
+    sync(s);
+  } catch (...) {
+    pmf_destroy(s);
+    throw;
+  }
+  *out = s;
+}
+}  // namespace
+
+int pmf_create(const pmf_params* pp, pmf_state** out) {
+  return guard([&] { create_state(pp, out, true); });
+}
+
+int pmf_create_workspace(const pmf_params* pp, pmf_state** out) {
+  return guard([&] { create_state(pp, out, false); });
 }
 
 int pmf_destroy(pmf_state* s) {
@@ -269,7 +312,7 @@ int pmf_destroy(pmf_state* s) {
 
 int pmf_upload(pmf_state* s, int32_t var, const void* host, int64_t n) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (var < 0 || var > 8) throw Error("variable index must be 0..8");
     if (n != s->nc) throw Error("host array has " + std::to_string(n) + " entries, expected " + std::to_string(s->nc));
     hip_check(hipMemcpyAsync((char*)s->U + var * s->nc * s->esz, host, n * s->esz, hipMemcpyHostToDevice, s->st),
@@ -281,7 +324,7 @@ int pmf_upload(pmf_state* s, int32_t var, const void* host, int64_t n) {
 
 int pmf_download(pmf_state* s, int32_t var, void* host, int64_t n) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (var < 0 || var > 8) throw Error("variable index must be 0..8");
     if (n != s->nc) throw Error("host array has " + std::to_string(n) + " entries, expected " + std::to_string(s->nc));
     hip_check(hipMemcpyAsync(host, (char*)s->U + var * s->nc * s->esz, n * s->esz, hipMemcpyDeviceToHost, s->st),
@@ -292,7 +335,7 @@ int pmf_download(pmf_state* s, int32_t var, void* host, int64_t n) {
 
 int pmf_device_ptr(pmf_state* s, int32_t var, void** ptr) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (var < 0 || var > 8 || !ptr) throw Error("variable index must be 0..8");
     *ptr = (char*)s->U + var * s->nc * s->esz;
     s->smax_valid = false;  // the caller may write through the pointer
@@ -301,7 +344,7 @@ int pmf_device_ptr(pmf_state* s, int32_t var, void** ptr) {
 
 int pmf_scratch_ptr(pmf_state* s, int32_t var, void** ptr) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (var < 0 || var > 8 || !ptr) throw Error("variable index must be 0..8");
     *ptr = (char*)s->S + var * s->nc * s->esz;
   });
@@ -346,14 +389,14 @@ int pmf_synchronize(pmf_state* s) {
 
 int pmf_max_wavespeed(pmf_state* s, double* smax) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     *smax = smax_device(s);
   });
 }
 
 int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     double sm = smax_device(s);
     if (s->p.precision == PMF_F64) {
       *dt = cfl * s->p.dx / std::max(sm, std::numeric_limits<double>::epsilon());
@@ -431,11 +474,12 @@ int pmf_max_divb(pmf_state* s, double* out) {
 
 int pmf_divb_norm_fp64(pmf_state* s, double* l2, double* linf) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (s->p.precision == PMF_F64)
-      launch_divb<double>((const double*)s->U, s->G, 1.0 / (2.0 * s->p.dx), s->red, s->dval + 1, s->st);
+      launch_divb<double>(cfields_at<double>(s->U, s->nc), s->G, 1.0 / (2.0 * s->p.dx), s->red, s->dval + 1, s->st);
     else
-      launch_divb<float>((const float*)s->U, s->G, 1.0f / (2.0f * float(s->p.dx)), s->red, s->dval + 1, s->st);
+      launch_divb<float>(cfields_at<float>(s->U, s->nc), s->G, 1.0f / (2.0f * float(s->p.dx)), s->red, s->dval + 1,
+                         s->st);
     hip_check(hipGetLastError(), "divb launch");
     fetch(s, 1, 2);
     if (l2) *l2 = std::sqrt(s->hval[1] / double(s->nc));
@@ -445,9 +489,9 @@ int pmf_divb_norm_fp64(pmf_state* s, double* l2, double* linf) {
 
 int pmf_conserved_totals_fp64(pmf_state* s, double out[8]) {
   return guard([&] {
-    check_state(s);
-    if (s->p.precision == PMF_F64) launch_totals<double>((const double*)s->U, s->G, s->red, s->dval + 4, s->st);
-    else launch_totals<float>((const float*)s->U, s->G, s->red, s->dval + 4, s->st);
+    check_fields(s);
+    if (s->p.precision == PMF_F64) launch_totals<double>(cfields_at<double>(s->U, s->nc), s->G, s->red, s->dval + 4, s->st);
+    else launch_totals<float>(cfields_at<float>(s->U, s->nc), s->G, s->red, s->dval + 4, s->st);
     hip_check(hipGetLastError(), "totals launch");
     fetch(s, 4, 8);
     double dV = s->p.dx * s->p.dx * s->p.dx;
@@ -471,14 +515,14 @@ int pmf_max_divb_face(pmf_state* s, double* out) {
 
 int pmf_ct_install_faces(pmf_state* s, const void* bx, const void* by, const void* bz) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (!s->ct) throw Error("_ct_install_faces! needs a CT state (mhd_solver=:ct)");
     size_t b = s->nc * s->esz;
     const void* src[3] = {bx, by, bz};
     for (int d = 0; d < 3; ++d)
       hip_check(hipMemcpyAsync((char*)s->ct + d * b, src[d], b, hipMemcpyHostToDevice, s->st), "upload faces");
-    if (s->p.precision == PMF_F64) launch_ct_install<double>((double*)s->U, (double*)s->ct, s->G, s->st);
-    else launch_ct_install<float>((float*)s->U, (float*)s->ct, s->G, s->st);
+    if (s->p.precision == PMF_F64) launch_ct_install<double>(fields_at<double>(s->U, s->nc), (double*)s->ct, s->G, s->st);
+    else launch_ct_install<float>(fields_at<float>(s->U, s->nc), (float*)s->ct, s->G, s->st);
     hip_check(hipGetLastError(), "ct_install launch");
     sync(s);
     s->smax_valid = false;
@@ -487,7 +531,7 @@ int pmf_ct_install_faces(pmf_state* s, const void* bx, const void* by, const voi
 
 int pmf_ct_faces_from_potential(pmf_state* s, const void* ax, const void* ay, const void* az) {
   return guard([&] {
-    check_state(s);
+    check_fields(s);
     if (!s->ct) throw Error("ct_faces_from_potential! needs a CT state (mhd_solver=:ct)");
     size_t b = s->nc * s->esz;
     // the potential is staged in scratch slots 0..2 (caller workspace between steps)
@@ -502,8 +546,8 @@ int pmf_ct_faces_from_potential(pmf_state* s, const void* ax, const void* ay, co
     else
       launch_ct_curl<float>((float*)S, (float*)(S + b), (float*)(S + 2 * b), (float*)F, (float*)(F + b),
                             (float*)(F + 2 * b), s->G, 1.0f / float(s->p.dx), s->st);
-    if (s->p.precision == PMF_F64) launch_ct_install<double>((double*)s->U, (double*)s->ct, s->G, s->st);
-    else launch_ct_install<float>((float*)s->U, (float*)s->ct, s->G, s->st);
+    if (s->p.precision == PMF_F64) launch_ct_install<double>(fields_at<double>(s->U, s->nc), (double*)s->ct, s->G, s->st);
+    else launch_ct_install<float>(fields_at<float>(s->U, s->nc), (float*)s->ct, s->G, s->st);
     hip_check(hipGetLastError(), "ct curl launch");
     sync(s);
     s->smax_valid = false;
@@ -568,6 +612,32 @@ int pmf_timing_read(pmf_state* s, double* total_ms, int64_t* launches) {
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = int64_t(s->ev_used / 2);
+  });
+}
+
+int pmf_step_into(pmf_state* s, const void* const* U, void* const* O, void* ct, double dt, double ch, double decay,
+                  int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    if (!U || !O) throw Error("null field array");
+    if (s->p.precision == PMF_F64)
+      step_fields<double>(s, cfields_from<double>(U), fields_from<double>(O), (double*)ct, dt, ch, decay, integrator);
+    else
+      step_fields<float>(s, cfields_from<float>(U), fields_from<float>(O), (float*)ct, dt, ch, decay, integrator);
+  });
+}
+
+int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax) {
+  return guard([&] {
+    check_state(s);
+    if (!U || !smax) throw Error("null argument");
+    if (s->p.precision == PMF_F64)
+      launch_wavespeed<double>(cfields_from<double>(U), s->G, make_params<double>(s, 0, 0, 1), s->red, s->dval, s->st);
+    else
+      launch_wavespeed<float>(cfields_from<float>(U), s->G, make_params<float>(s, 0, 0, 1), s->red, s->dval, s->st);
+    hip_check(hipGetLastError(), "wavespeed launch");
+    fetch(s, 0, 1);
+    *smax = s->hval[0];
   });
 }
 

@@ -12,27 +12,29 @@ constexpr int kRedBlocks = 2048;
 constexpr int kRedSlots = 24;  // doubles per block slot
 
 template <class T>
-void launch_step_ref(const T* U, T* O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct, hipStream_t st);
+void launch_step_ref(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct,
+                     hipStream_t st);
 
 // Fused tiled step (periodic GLM). Writes per-block CFL maxima of the new state into
 // red (one double per block) and the final max into dsmax. Returns false when the grid
 // shape is not supported by the fused kernel.
 template <class T>
-bool launch_step_fused(const T* U, T* O, const Grid& G, const StepParams<T>& P, double* red, double* dsmax,
-                       hipStream_t st);
+bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
+                       double* dsmax, hipStream_t st);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
 // Number of workgroups (one partial maximum each) the fused kernel launches for G.
 int64_t fused_blocks(const Grid& G);
 
 template <class T>
-void launch_wavespeed(const T* U, const Grid& G, const StepParams<T>& P, double* red, double* out, hipStream_t st);
+void launch_wavespeed(Fields<const T> U, const Grid& G, const StepParams<T>& P, double* red, double* out,
+                      hipStream_t st);
 template <class T>
-void launch_divb(const T* U, const Grid& G, T inv2dx, double* red, double* out3, hipStream_t st);
-template <class T> void launch_totals(const T* U, const Grid& G, double* red, double* out8, hipStream_t st);
-template <class T> void launch_ct_update(T* U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st);
+void launch_divb(Fields<const T> U, const Grid& G, T inv2dx, double* red, double* out3, hipStream_t st);
+template <class T> void launch_totals(Fields<const T> U, const Grid& G, double* red, double* out8, hipStream_t st);
+template <class T> void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st);
 template <class T> void launch_divb_face(const T* ctp, const Grid& G, T invdx, double* red, double* out, hipStream_t st);
 template <class T>
-void launch_ct_install(T* U, T* ctp, const Grid& G, hipStream_t st);
+void launch_ct_install(Fields<T> U, T* ctp, const Grid& G, hipStream_t st);
 template <class T>
 void launch_ct_curl(const T* Ax, const T* Ay, const T* Az, T* bx, T* by, T* bz, const Grid& G, T invdx,
                     hipStream_t st);

@@ -26,6 +26,12 @@ template <class T> struct StepParams {
   int32_t recon, rsolve;
 };
 
+// The nine conserved fields, each a flat array of nc cells (state.jl:63-65: one array per
+// variable, so a caller's separately allocated arrays are used in place).
+template <class T> struct Fields {
+  T* v[9];
+};
+
 struct Grid {
   int32_t nx, ny, nz;
   int32_t bc[3];

@@ -109,14 +109,14 @@ __device__ __forceinline__ void store_partial(double* red, KSum* sums, double* m
 
 // ---- CFL wave speed (timestep.jl:4-23) ------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlk) void k_wavespeed(const T* __restrict__ U, Grid G, StepParams<T> P,
+__global__ __launch_bounds__(kBlk) void k_wavespeed(Fields<const T> U, Grid G, StepParams<T> P,
                                                     double* __restrict__ red) {
   double m = 0.0;
   const int64_t n = G.nc;
   for (int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x; c < n; c += (int64_t)gridDim.x * kBlk) {
     T q[9];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) q[v] = U[v * n + c];
+    for (int v = 0; v < 9; ++v) q[v] = U.v[v][c];
     m = nanmax(m, (double)cell_wavespeed(q, P));
   }
   KSum none[1];
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(kBlk) void k_wavespeed(const T* __restrict__ U, Gri
 }
 
 template <class T>
-void launch_wavespeed(const T* U, const Grid& G, const StepParams<T>& P, double* red, double* out, hipStream_t st) {
+void launch_wavespeed(Fields<const T> U, const Grid& G, const StepParams<T>& P, double* red, double* out,
+                      hipStream_t st) {
   unsigned nb = red_grid(G.nc);
   hipLaunchKernelGGL((k_wavespeed<T>), dim3(nb), dim3(kBlk), 0, st, U, G, P, red);
   hipLaunchKernelGGL((k_fold_final<0, 1>), dim3(1), dim3(kBlk), 0, st, red, (int)nb, out);
@@ -148,9 +149,9 @@ __device__ __forceinline__ int bc_idx(int i, int n, int code, double& flip) {
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlk) void k_divb(const T* __restrict__ U, Grid G, T inv2dx, double* __restrict__ red) {
+__global__ __launch_bounds__(kBlk) void k_divb(Fields<const T> U, Grid G, T inv2dx, double* __restrict__ red) {
   const int64_t n = G.nc;
-  const T *bx = U + 5 * n, *by = U + 6 * n, *bz = U + 7 * n;
+  const T *bx = U.v[5], *by = U.v[6], *bz = U.v[7];
   double m = 0.0;
   KSum s{0.0, 0.0};
   for (int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x; c < n; c += (int64_t)gridDim.x * kBlk) {
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kBlk) void k_divb(const T* __restrict__ U, Grid G, 
 }
 
 template <class T>
-void launch_divb(const T* U, const Grid& G, T inv2dx, double* red, double* out, hipStream_t st) {
+void launch_divb(Fields<const T> U, const Grid& G, T inv2dx, double* red, double* out, hipStream_t st) {
   unsigned nb = red_grid(G.nc);
   hipLaunchKernelGGL((k_divb<T>), dim3(nb), dim3(kBlk), 0, st, U, G, inv2dx, red);
   hipLaunchKernelGGL((k_fold_final<1, 1>), dim3(1), dim3(kBlk), 0, st, red, (int)nb, out);
@@ -179,21 +180,21 @@ void launch_divb(const T* U, const Grid& G, T inv2dx, double* red, double* out, 
 
 // ---- conserved totals (diagnostics.jl:44-55) -------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlk) void k_totals(const T* __restrict__ U, Grid G, double* __restrict__ red) {
+__global__ __launch_bounds__(kBlk) void k_totals(Fields<const T> U, Grid G, double* __restrict__ red) {
   const int64_t n = G.nc;
   KSum s[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) s[v] = KSum{0.0, 0.0};
   for (int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x; c < n; c += (int64_t)gridDim.x * kBlk) {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) s[v] = ksum_add(s[v], (double)U[v * n + c]);
+    for (int v = 0; v < 8; ++v) s[v] = ksum_add(s[v], (double)U.v[v][c]);
   }
   double none[1];
   block_fold<8, 0>(s, none);
   store_partial<8, 0>(red, s, none);
 }
 
-template <class T> void launch_totals(const T* U, const Grid& G, double* red, double* out8, hipStream_t st) {
+template <class T> void launch_totals(Fields<const T> U, const Grid& G, double* red, double* out8, hipStream_t st) {
   unsigned nb = red_grid(G.nc);
   hipLaunchKernelGGL((k_totals<T>), dim3(nb), dim3(kBlk), 0, st, U, G, red);
   hipLaunchKernelGGL((k_fold_final<8, 0>), dim3(1), dim3(kBlk), 0, st, red, (int)nb, out8);
@@ -270,18 +271,18 @@ template <class T, bool COMP> __global__ __launch_bounds__(kBlk) void k_ct_faces
 }
 
 // ct.jl:132-143: cell B = face average.
-template <class T> __global__ __launch_bounds__(kBlk) void k_ct_bcc(T* __restrict__ U, const T* __restrict__ ctp, Grid G) {
+template <class T> __global__ __launch_bounds__(kBlk) void k_ct_bcc(Fields<T> U, const T* __restrict__ ctp, Grid G) {
   const int64_t n = G.nc;
   int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
   if (c >= n) return;
   Pc p{G.nx, G.ny, G.nz};
   int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
-  U[5 * n + c] = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
-  U[6 * n + c] = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
-  U[7 * n + c] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
+  U.v[5][c] = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
+  U.v[6][c] = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
+  U.v[7][c] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
 }
 
-template <class T> void launch_ct_update(T* U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st) {
+template <class T> void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st) {
   dim3 blk(kBlk), grd((unsigned)((G.nc + kBlk - 1) / kBlk));
   hipLaunchKernelGGL((k_ct_emf<T>), grd, blk, 0, st, ctp, G);
   if (compensate) hipLaunchKernelGGL((k_ct_faces<T, true>), grd, blk, 0, st, ctp, G, dtdx);
@@ -313,7 +314,7 @@ template <class T> void launch_divb_face(const T* ctp, const Grid& G, T invdx, d
 }
 
 // ct.jl:205-219: cell B <- face average, E keeps the thermal pressure, psi <- 0.
-template <class T> __global__ __launch_bounds__(kBlk) void k_ct_install(T* __restrict__ U, const T* __restrict__ ctp, Grid G) {
+template <class T> __global__ __launch_bounds__(kBlk) void k_ct_install(Fields<T> U, const T* __restrict__ ctp, Grid G) {
   const int64_t n = G.nc;
   int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
   if (c >= n) return;
@@ -322,12 +323,12 @@ template <class T> __global__ __launch_bounds__(kBlk) void k_ct_install(T* __res
   T bx = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
   T by = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
   T bz = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
-  T ox = U[5 * n + c], oy = U[6 * n + c], oz = U[7 * n + c];
-  U[4 * n + c] += T(0.5) * ((bx * bx + by * by + bz * bz) - (ox * ox + oy * oy + oz * oz));
-  U[5 * n + c] = bx;
-  U[6 * n + c] = by;
-  U[7 * n + c] = bz;
-  U[8 * n + c] = T(0);
+  T ox = U.v[5][c], oy = U.v[6][c], oz = U.v[7][c];
+  U.v[4][c] += T(0.5) * ((bx * bx + by * by + bz * bz) - (ox * ox + oy * oy + oz * oz));
+  U.v[5][c] = bx;
+  U.v[6][c] = by;
+  U.v[7][c] = bz;
+  U.v[8][c] = T(0);
 }
 
 template <class T> __global__ __launch_bounds__(kBlk) void k_zero(T* __restrict__ a, int64_t n) {
@@ -335,7 +336,7 @@ template <class T> __global__ __launch_bounds__(kBlk) void k_zero(T* __restrict_
   if (c < n) a[c] = T(0);
 }
 
-template <class T> void launch_ct_install(T* U, T* ctp, const Grid& G, hipStream_t st) {
+template <class T> void launch_ct_install(Fields<T> U, T* ctp, const Grid& G, hipStream_t st) {
   dim3 blk(kBlk), grd((unsigned)((G.nc + kBlk - 1) / kBlk)), grz((unsigned)((3 * G.nc + kBlk - 1) / kBlk));
   hipLaunchKernelGGL((k_zero<T>), grz, blk, 0, st, ctp + 18 * G.nc, 3 * G.nc);
   hipLaunchKernelGGL((k_ct_install<T>), grd, blk, 0, st, U, ctp, G);
@@ -363,12 +364,13 @@ void launch_ct_curl(const T* Ax, const T* Ay, const T* Az, T* bx, T* by, T* bz, 
 }
 
 #define PMF_INST(T)                                                                                             \
-  template void launch_wavespeed<T>(const T*, const Grid&, const StepParams<T>&, double*, double*, hipStream_t); \
-  template void launch_divb<T>(const T*, const Grid&, T, double*, double*, hipStream_t);                          \
-  template void launch_totals<T>(const T*, const Grid&, double*, double*, hipStream_t);                          \
-  template void launch_ct_update<T>(T*, T*, const Grid&, T, bool, hipStream_t);                                  \
+  template void launch_wavespeed<T>(Fields<const T>, const Grid&, const StepParams<T>&, double*, double*,       \
+                                    hipStream_t);                                                               \
+  template void launch_divb<T>(Fields<const T>, const Grid&, T, double*, double*, hipStream_t);                  \
+  template void launch_totals<T>(Fields<const T>, const Grid&, double*, double*, hipStream_t);                  \
+  template void launch_ct_update<T>(Fields<T>, T*, const Grid&, T, bool, hipStream_t);                           \
   template void launch_divb_face<T>(const T*, const Grid&, T, double*, double*, hipStream_t);                    \
-  template void launch_ct_install<T>(T*, T*, const Grid&, hipStream_t);                                          \
+  template void launch_ct_install<T>(Fields<T>, T*, const Grid&, hipStream_t);                                   \
   template void launch_ct_curl<T>(const T*, const T*, const T*, T*, T*, T*, const Grid&, T, hipStream_t);
 PMF_INST(float)
 PMF_INST(double)

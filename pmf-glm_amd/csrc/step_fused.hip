@@ -46,9 +46,10 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 }
 
 template <class T>
-__device__ __forceinline__ void load_cons(const T* __restrict__ U, int64_t idx, int64_t nc, T* c) {
+__device__ __forceinline__ void load_cons(const Fields<const T>& U, int64_t idx, int64_t nc, T* c) {
+  (void)nc;
 #pragma unroll
-  for (int v = 0; v < 9; ++v) c[v] = U[v * nc + idx];
+  for (int v = 0; v < 9; ++v) c[v] = U.v[v][idx];
 }
 
 // Ring cell h (0..NRING-1) at distance r from the tile: bottom row, top row, left
@@ -96,7 +97,7 @@ __device__ __forceinline__ int pidx(int ix, int iy) { return (iy + 2) * FPW + (i
 __device__ __forceinline__ int qidx(int ix, int iy) { return (iy + 1) * FQW + (ix + 1); }
 
 template <class T, int RECON, int RS>
-__global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U, T* __restrict__ O, FusedGeom g,
+__global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
                                                          StepParams<T> P, double* __restrict__ red) {
   __shared__ T sP[2][9 * FNP];    // primitive planes m (current) and m-1, alternating
   __shared__ T sLX[9 * FNQ];      // +x face states, then x-face fluxes
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(const T* __restrict__ U
           for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
           o[8] *= P.decay;
 #pragma unroll
-          for (int v = 0; v < 9; ++v) O[v * g.nc + idx] = o[v];
+          for (int v = 0; v < 9; ++v) O.v[v][idx] = o[v];
           smax = nanmax(smax, cell_wavespeed(o, P));
         }
         if (!first) {
@@ -355,8 +356,8 @@ int64_t fused_blocks(const Grid& G) {
 }
 
 template <class T>
-bool launch_step_fused(const T* U, T* O, const Grid& G, const StepParams<T>& P, double* red, double* dsmax,
-                       hipStream_t st) {
+bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
+                       double* dsmax, hipStream_t st) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
@@ -379,9 +380,9 @@ bool launch_step_fused(const T* U, T* O, const Grid& G, const StepParams<T>& P, 
   return true;
 }
 
-template bool launch_step_fused<float>(const float*, float*, const Grid&, const StepParams<float>&, double*, double*,
-                                       hipStream_t);
-template bool launch_step_fused<double>(const double*, double*, const Grid&, const StepParams<double>&, double*,
-                                        double*, hipStream_t);
+template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
+                                       double*, double*, hipStream_t);
+template bool launch_step_fused<double>(Fields<const double>, Fields<double>, const Grid&, const StepParams<double>&,
+                                        double*, double*, hipStream_t);
 
 }  // namespace pmf

@@ -27,22 +27,21 @@ __device__ __forceinline__ int bc_map(int i, int n, int code, float& flip) {
 }
 
 template <class T>
-__device__ __forceinline__ void load_prim(const T* __restrict__ U, const Grid& G, int i, int j, int k, T* q,
+__device__ __forceinline__ void load_prim(const Fields<const T>& U, const Grid& G, int i, int j, int k, T* q,
                                           const StepParams<T>& P) {
   float fx, fy, fz;
   int ii = bc_map(i, G.nx, G.bc[0], fx), jj = bc_map(j, G.ny, G.bc[1], fy), kk = bc_map(k, G.nz, G.bc[2], fz);
   int64_t idx = ((int64_t)kk * G.ny + jj) * G.nx + ii;
-  const int64_t n = G.nc;
   T c[9];
-  c[0] = U[idx];
-  c[1] = U[n + idx] * T(fx);
-  c[2] = U[2 * n + idx] * T(fy);
-  c[3] = U[3 * n + idx] * T(fz);
-  c[4] = U[4 * n + idx];
-  c[5] = U[5 * n + idx] * T(fx);
-  c[6] = U[6 * n + idx] * T(fy);
-  c[7] = U[7 * n + idx] * T(fz);
-  c[8] = U[8 * n + idx];
+  c[0] = U.v[0][idx];
+  c[1] = U.v[1][idx] * T(fx);
+  c[2] = U.v[2][idx] * T(fy);
+  c[3] = U.v[3][idx] * T(fz);
+  c[4] = U.v[4][idx];
+  c[5] = U.v[5][idx] * T(fx);
+  c[6] = U.v[6][idx] * T(fy);
+  c[7] = U.v[7][idx] * T(fz);
+  c[8] = U.v[8][idx];
   cons2prim(c, q, P);
 }
 
@@ -55,7 +54,7 @@ __device__ __forceinline__ void offsets9(const T* qm, const T* q0, const T* qp, 
 // Hancock prediction of logical cell (i,j,k) (integrator_ref.jl:17-26): mh and the
 // per-axis face offsets.
 template <int RECON, class T>
-__device__ void predict_cell(const T* __restrict__ U, const Grid& G, int i, int j, int k, const StepParams<T>& P,
+__device__ void predict_cell(const Fields<const T>& U, const Grid& G, int i, int j, int k, const StepParams<T>& P,
                              T* mh, T (*dL)[9], T (*dR)[9]) {
   T m0[9], qm[9], qp[9];
   load_prim(U, G, i, j, k, m0, P);
@@ -103,7 +102,7 @@ __device__ void predict_cell(const T* __restrict__ U, const Grid& G, int i, int 
 
 // Both faces of one axis for the cell: D[v] = F_lo - F_hi (integrator_ref.jl:74-125).
 template <int D, int RECON, bool CT, class T>
-__device__ void axis_faces(const T* __restrict__ U, T* __restrict__ ctp, const Grid& G, int i, int j, int k,
+__device__ void axis_faces(const Fields<const T>& U, T* __restrict__ ctp, const Grid& G, int i, int j, int k,
                            int64_t c, const T* mhc, const T (*dLc)[9], const T (*dRc)[9], const StepParams<T>& P,
                            T* Dd) {
   constexpr int di = D == 0, dj = D == 1, dk = D == 2;
@@ -162,8 +161,8 @@ __device__ void axis_faces(const T* __restrict__ U, T* __restrict__ ctp, const G
 }
 
 template <int RECON, bool CT, class T>
-__global__ __launch_bounds__(256) void k_step_ref(const T* __restrict__ U, T* __restrict__ O, T* __restrict__ ctp,
-                                                  Grid G, StepParams<T> P) {
+__global__ __launch_bounds__(256) void k_step_ref(Fields<const T> U, Fields<T> O, T* __restrict__ ctp, Grid G,
+                                                  StepParams<T> P) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= G.nc) return;
   int i = (int)(c % G.nx);
@@ -178,7 +177,7 @@ __global__ __launch_bounds__(256) void k_step_ref(const T* __restrict__ U, T* __
   const int64_t n = G.nc;
   T U0[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) U0[v] = U[v * n + c];
+  for (int v = 0; v < 9; ++v) U0[v] = U.v[v][c];
   if constexpr (CT) {
     // integrator_ref.jl:133-138: E = -v x B with the predicted velocity and the
     // start-of-step cell field
@@ -188,12 +187,13 @@ __global__ __launch_bounds__(256) void k_step_ref(const T* __restrict__ U, T* __
   }
   // integrator_ref.jl:140-158
 #pragma unroll
-  for (int v = 0; v < 8; ++v) O[v * n + c] = U0[v] + P.dtdx * ((Dx[v] + Dy[v]) + Dz[v]);
-  O[8 * n + c] = CT ? T(0) : (U0[8] + P.dtdx * ((Dx[8] + Dy[8]) + Dz[8])) * P.decay;
+  for (int v = 0; v < 8; ++v) O.v[v][c] = U0[v] + P.dtdx * ((Dx[v] + Dy[v]) + Dz[v]);
+  O.v[8][c] = CT ? T(0) : (U0[8] + P.dtdx * ((Dx[8] + Dy[8]) + Dz[8])) * P.decay;
 }
 
 template <class T>
-void launch_step_ref(const T* U, T* O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct, hipStream_t st) {
+void launch_step_ref(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct,
+                     hipStream_t st) {
   dim3 blk(256), grd((unsigned)((G.nc + 255) / 256));
 #define PMF_REF(R, C) hipLaunchKernelGGL((k_step_ref<R, C, T>), grd, blk, 0, st, U, O, ctp, G, P)
   if (ct) {
@@ -235,10 +235,10 @@ void launch_riemann_batch(const T* L, const T* R, T* F, int64_t n, int dir, bool
   else hipLaunchKernelGGL((k_riemann_batch<false, T>), grd, blk, 0, st, L, R, F, n, dir, P);
 }
 
-template void launch_step_ref<float>(const float*, float*, float*, const Grid&, const StepParams<float>&, bool,
-                                     hipStream_t);
-template void launch_step_ref<double>(const double*, double*, double*, const Grid&, const StepParams<double>&, bool,
-                                      hipStream_t);
+template void launch_step_ref<float>(Fields<const float>, Fields<float>, float*, const Grid&,
+                                     const StepParams<float>&, bool, hipStream_t);
+template void launch_step_ref<double>(Fields<const double>, Fields<double>, double*, const Grid&,
+                                      const StepParams<double>&, bool, hipStream_t);
 template void launch_riemann_batch<float>(const float*, const float*, float*, int64_t, int, bool,
                                           const StepParams<float>&, hipStream_t);
 template void launch_riemann_batch<double>(const double*, const double*, double*, int64_t, int, bool,

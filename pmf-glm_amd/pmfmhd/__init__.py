@@ -22,7 +22,7 @@ __all__ = ["GLM_CR", "MHDState", "MHDError", "allocate_state", "step", "step_ref
            "compute_dt", "max_wavespeed", "max_divb", "divb_norm", "conserved_totals", "max_divb_face",
            "ct_faces_from_cells", "ct_faces_from_potential", "riemann_flux", "fields_to_host",
            "init_alfven_wave", "init_orszag_tang", "init_brio_wu", "init_turb_field", "init_field_loop",
-           "init_config", "problems"]
+           "init_config", "problems", "step_into", "max_wavespeed_of"]
 
 GLM_CR = 0.18  # driver.jl:8
 _RIEMANN = {"hlld": 0, "hll": 1, "llf": 2}
@@ -39,14 +39,17 @@ def _code(table, kind, value, have):
 
 
 class MHDState:
-    """Device state: 9 SoA conserved fields, scratch set and, for CT, the face buffer."""
+    """Device state: 9 SoA conserved fields, scratch set and, for CT, the face buffer.
+    With ``workspace=True`` it holds no fields (the caller owns its arrays; see step_into)."""
 
-    def __init__(self, params: pmf_params, dtype, riemann, recon, bcs, solver):
+    def __init__(self, params: pmf_params, dtype, riemann, recon, bcs, solver, workspace=False):
         self._p = params
         self.dtype = np.dtype(dtype)
         self.riemann, self.recon, self.bcs, self._solver = riemann, recon, bcs, solver
+        self.workspace = workspace
         h = ctypes.c_void_p()
-        check(lib().pmf_create(ctypes.byref(params), ctypes.byref(h)))
+        create = lib().pmf_create_workspace if workspace else lib().pmf_create
+        check(create(ctypes.byref(params), ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -137,8 +140,10 @@ class MHDState:
 
 def allocate_state(dims, dx, dtype=np.float32, gamma=5.0 / 3.0, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4,
                    llf_pmin=0.0, use_hlld=True, riemann=None, bc="periodic", bcs=None, recon="plm",
-                   mhd_solver="glm") -> MHDState:
-    """state.jl:109-141. ``dtype`` is np.float32 (the bulk precision) or np.float64."""
+                   mhd_solver="glm", workspace=False) -> MHDState:
+    """state.jl:109-141. ``dtype`` is np.float32 (the bulk precision) or np.float64.
+    ``workspace=True`` allocates no fields: the state then steps caller-owned arrays
+    through ``step_into`` / ``max_wavespeed_of``."""
     if mhd_solver not in ("glm", "ct"):
         raise MHDError(f"unknown mhd_solver :{mhd_solver} (have :glm, :ct)")
     bcs = tuple(bcs) if bcs is not None else (bc, bc, bc)
@@ -160,7 +165,7 @@ def allocate_state(dims, dx, dtype=np.float32, gamma=5.0 / 3.0, smallr=1e-6, pfl
     if dt not in (np.float32, np.float64):
         raise MHDError("dtype must be float32 or float64")
     p.precision = 1 if dt == np.float64 else 0
-    return MHDState(p, dt, rsym, recon, bcs, mhd_solver)
+    return MHDState(p, dt, rsym, recon, bcs, mhd_solver, workspace=workspace)
 
 
 def fields_to_host(s: MHDState):
@@ -196,6 +201,25 @@ def step_cube(s: MHDState, dt, ch, decay):
                        "(mhd_solver=:ct states cannot use the :cube integrators)")
     check(lib().pmf_step_raw(s.handle, float(dt), float(ch), float(decay), 1))
     return s
+
+
+def _ptrs(arrays):
+    if len(arrays) != 9:
+        raise MHDError("nine field arrays are required")
+    return (ctypes.c_void_p * 9)(*[int(a) for a in arrays])
+
+
+def step_into(ws: MHDState, U_ptrs, O_ptrs, dt, ch, decay, integrator="ref", ct_ptr=None):
+    """The _cube_launch! seam: one step from nine caller device arrays into nine others (no
+    swap), asynchronous on the workspace's stream. Pointers are integer device addresses."""
+    check(lib().pmf_step_into(ws.handle, _ptrs(U_ptrs), _ptrs(O_ptrs), ct_ptr, float(dt), float(ch), float(decay),
+                              _integ(integrator)))
+
+
+def max_wavespeed_of(ws: MHDState, U_ptrs):
+    v = ctypes.c_double()
+    check(lib().pmf_max_wavespeed_of(ws.handle, _ptrs(U_ptrs), ctypes.byref(v)))
+    return ws.dtype.type(v.value)
 
 
 def max_wavespeed(s: MHDState):

@@ -34,6 +34,9 @@ def lib():
             "pmf_params_default": [ctypes.POINTER(pmf_params), i32, i32, i32, d],
             "pmf_create": [ctypes.POINTER(pmf_params), pp],
             "pmf_destroy": [P],
+            "pmf_create_workspace": [ctypes.POINTER(pmf_params), pp],
+            "pmf_step_into": [P, P, P, P, d, d, d, i32],
+            "pmf_max_wavespeed_of": [P, P, pd],
             "pmf_upload": [P, i32, P, i64],
             "pmf_download": [P, i32, P, i64],
             "pmf_device_ptr": [P, i32, pp],

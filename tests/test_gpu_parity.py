@@ -306,3 +306,30 @@ def test_fused_rejects_unsupported_grids():
     b = M.allocate_state((16, 16, 8), 1 / 16, bcs=("outflow", "periodic", "periodic"))
     with pytest.raises(M.MHDError, match="all-periodic"):
         M.step(b, 1e-3, ch=1.0, integrator="cube")
+
+
+@pytest.mark.parametrize("integrator", ["ref", "cube"])
+def test_step_into_caller_owned_arrays_is_bitwise_the_state_step(integrator):
+    """The drop-in seam: nine separately allocated device arrays (as the reference's
+    MHDState holds them), stepped through a field-less workspace."""
+    import torch
+    dims = (32, 24, 10)
+    U = smooth_state(dims, seed=21).astype(np.float32)
+    s = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32)
+    s.set_conserved(U)
+    dt, smax = M.compute_dt(s, 0.3)
+    decay = float(np.exp(np.float32(-M.GLM_CR) * smax * dt / np.float32(1.0 / dims[0])))
+    M.step_ref(s, float(dt), float(smax), decay) if integrator == "ref" else M.step_cube(s, float(dt), float(smax), decay)
+    want = s.conserved()
+    ins = [torch.from_numpy(U[v].copy()).to("cuda") for v in range(9)]
+    outs = [torch.empty_like(ins[v]) for v in range(9)]
+    torch.cuda.synchronize()
+    ws = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32, workspace=True)
+    assert float(M.max_wavespeed_of(ws, [t.data_ptr() for t in ins])) == float(smax)
+    M.step_into(ws, [t.data_ptr() for t in ins], [t.data_ptr() for t in outs], float(dt), float(smax), decay,
+                integrator=integrator)
+    ws.synchronize()
+    got = np.stack([t.cpu().numpy() for t in outs])
+    assert np.array_equal(got, want)
+    with pytest.raises(M.MHDError, match="workspace without fields"):
+        M.compute_dt(ws)

@@ -358,3 +358,43 @@ def test_ct_divergence_preserved_under_evolution():
     assert info["max_divb_face_dx_over_brms"] < 1e-12
     assert max(divs) < 1e-12
     assert np.all(s.U[8] == 0)
+
+
+def test_reflecting_wall_equals_mirrored_periodic_domain():
+    """boundary.jl:17-31: a reflecting wall is a mirror. A half domain with reflecting x walls
+    advances exactly like the left half of a periodic domain holding the state and its mirror
+    image (normal velocity and normal field flipped)."""
+    nx, ny, nz = 8, 6, 5
+    U = smooth_state(8, seed=9).reshape(9, 8, 8, 8)[:, :nz, :ny, :nx]
+    U = np.ascontiguousarray(U)
+    flip = np.array([1, -1, 1, 1, 1, -1, 1, 1, 1.0])[:, None, None, None]
+    Uf = np.concatenate([U, (U * flip)[:, :, :, ::-1]], axis=3)  # periodic in x with period 2 nx
+    a = O.OracleState((nx, ny, nz), 1.0 / 8, bcs=("reflecting", "periodic", "periodic"), recon="ppm")
+    b = O.OracleState((2 * nx, ny, nz), 1.0 / 8, recon="ppm")
+    a.set_conserved(U.reshape(9, -1))
+    b.set_conserved(Uf.reshape(9, -1))
+    dt, smax = b.compute_dt(0.3)
+    for _ in range(3):
+        a.step(dt, ch=smax)
+        b.step(dt, ch=smax)
+    Ub = b.U.reshape(9, nz, ny, 2 * nx)[:, :, :, :nx].reshape(9, -1)
+    assert np.max(np.abs(a.U - Ub)) < 1e-13
+
+
+def test_llf_consistency_and_minmod_convergence():
+    """LLF(U,U) is the physical flux (glm_physics.jl:239-244); minmod PLM converges at
+    second order on the reference's Alfven study (runtests.jl:6-25)."""
+    rng = np.random.default_rng(6)
+    Q = rand_prim(rng, 300, rho_lo=0.5)
+    for d in range(3):
+        Fx = O.riemann_batch(Q, Q, d, 1.3, riemann="llf")
+        qr = Q[:, F.ROT_TO[d]]
+        phys = F.phys_flux_x(qr.T, G).T
+        phys[:, 5] = qr[:, 8]
+        phys[:, 8] = 1.3 * 1.3 * qr[:, 5]
+        assert np.max(np.abs(Fx - phys[:, F.ROT_FROM[d]])) < 1e-12
+    L1 = [alfven_L1(N, recon="plm_minmod", dtype=np.float64)[0] for N in (32, 64, 128)]
+    orders = [np.log2(L1[k - 1] / L1[k]) for k in (1, 2)]
+    assert min(orders) > 1.6, orders
+    # minmod is more diffusive than MonCen on the same wave (validation-cpu.log:8-12)
+    assert L1[0] > 6.919e-4
