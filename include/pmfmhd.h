@@ -129,6 +129,41 @@ int pmf_riemann_flux_batch(pmf_state* s, const void* L, const void* R, int32_t d
 int pmf_timing_enable(pmf_state* s, int32_t on);
 int pmf_timing_read(pmf_state* s, double* total_ms, int64_t* launches);
 
+/* ---- Learned face flux (a BF16 MLP on MFMA standing in for HLLD on smooth faces) ----
+ * No counterpart in the reference: it adds a learned flux on top of the MHDKernels
+ * step. Network 16 -> 64 -> 64 -> 64 -> 9 (tanh GELU), float32 master weights, bfloat16
+ * operands with float32 accumulation. Inputs: the face's mean and jump of the 8
+ * normal-frame primitives, normalised by the local fast-speed scale (jumps divided by the
+ * gate threshold); outputs: the HLLD dissipation relative to the central flux. The gate
+ * (jump detector) admits a face when the relative jumps of rho, p and |B| are below thr. */
+typedef struct pmf_surrogate pmf_surrogate;
+int pmf_surrogate_create(pmf_state* s, uint64_t seed, pmf_surrogate** out);  /* s: float32; stream + params */
+int pmf_surrogate_destroy(pmf_surrogate* n);
+int pmf_surrogate_nparams(void);                                              /* 9993 */
+int pmf_surrogate_get_weights(pmf_surrogate* n, float* host, int64_t count);  /* [W1 b1 W2 b2 W3 b3 W4 b4] */
+int pmf_surrogate_set_weights(pmf_surrogate* n, const float* host, int64_t count);
+/* Training set on the device: capacity, dump from a state (every gated lower face of
+ * every cell kept with probability keep; dt, ch as for the step), copy out / upload.
+ * Rows: x[16], y[9], state[18] = normal-frame L[8], R[8], GLM flux of Bn, of psi. */
+int pmf_surrogate_reserve(pmf_surrogate* n, int64_t capacity);
+int pmf_surrogate_dump(pmf_surrogate* n, pmf_state* s, double dt, double ch, double thr, double keep,
+                       uint64_t seed, int64_t* total);
+int pmf_surrogate_dataset(pmf_surrogate* n, int64_t first, int64_t count, float* x, float* y, float* state);
+int pmf_surrogate_upload(pmf_surrogate* n, const float* x, const float* y, const float* state, int64_t count);
+/* Full-batch loss gradient over a dataset range (bf16 = 1: bfloat16 MFMA, 0: float32 MFMA). */
+int pmf_surrogate_grad(pmf_surrogate* n, int64_t first, int64_t count, int32_t bf16, float* grad, double* loss);
+int pmf_surrogate_predict(pmf_surrogate* n, int64_t first, int64_t count, int32_t bf16, float* y);
+/* Adam on hashed mini-batches; losses (optional) receives the loss of every step. */
+int pmf_surrogate_train(pmf_surrogate* n, int32_t batch, int32_t steps, double lr, uint64_t seed, int32_t bf16,
+                        double* losses);
+/* Face fluxes of dataset rows: mode 0 HLLD, 1 surrogate, 2 central flux only. */
+int pmf_surrogate_face_flux(pmf_surrogate* n, int64_t first, int64_t count, int32_t mode, double thr, float* F);
+/* step! with the fused kernel in which every face whose jumps pass the gate (thr) takes
+ * the learned flux and every other face HLLD (float32, GLM, HLLD, periodic, nx % 16 == 0).
+ * nsmooth (optional; waits for the stream) receives the number of network-solved faces. */
+int pmf_step_surrogate(pmf_state* s, pmf_surrogate* n, double dt, double ch, double glm_cr, double thr,
+                       int64_t* nsmooth);
+
 const char* pmf_last_error(void);
 const char* pmf_version(void);
 

@@ -218,8 +218,9 @@ def riemann(Lq, Rq, d, ch, g, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4, llf_pmin=0.0
 
 
 def step_periodic(U, dims, dtdx, ch, decay, g=5.0 / 3.0, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4, llf_pmin=0.0,
-                  recon="plm", solver="hlld"):
-    """One GLM step of a periodic grid; U is (9, nc) conserved, i fastest."""
+                  recon="plm", solver="hlld", flux=None):
+    """One GLM step of a periodic grid; U is (9, nc) conserved, i fastest. ``flux(L, R, d)``
+    replaces the Riemann solver when given (lab-frame face states in, lab-frame flux out)."""
     nx, ny, nz = dims
     u = U.reshape(9, nz, ny, nx)
     ax = (3, 2, 1)  # numpy axis of x, y, z in the (9, nz, ny, nx) layout
@@ -235,7 +236,10 @@ def step_periodic(U, dims, dtdx, ch, decay, g=5.0 / 3.0, smallr=1e-6, pfl=1e-7, 
         # lower face of every cell: L = mh(i-1) + dR(i-1), R = mh(i) + dL(i)
         L = np.roll(mh + dR[d], 1, axis=ax[d])
         R = mh + dL[d]
-        F = riemann(L, R, d, ch, g, smallr, pfl, llf_dmin, llf_pmin, solver)
+        if flux is None:
+            F = riemann(L, R, d, ch, g, smallr, pfl, llf_dmin, llf_pmin, solver)
+        else:
+            F = flux(L, R, d)
         dU.append(F - np.roll(F, -1, axis=ax[d]))
     out = u + dtdx * ((dU[0] + dU[1]) + dU[2])
     out[8] = out[8] * decay

@@ -22,6 +22,11 @@ template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
                        double* dsmax, hipStream_t st);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
+// The surrogate-assisted fused step (float32, HLLD): simg is the network's LDS image,
+// nsmooth accumulates the number of faces solved by the network.
+bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
+                            double* red, double* dsmax, const unsigned char* simg, float thr,
+                            unsigned long long* nsmooth, hipStream_t st);
 // Number of workgroups (one partial maximum each) the fused kernel launches for G.
 int64_t fused_blocks(const Grid& G);
 

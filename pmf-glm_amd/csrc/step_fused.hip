@@ -21,6 +21,7 @@
 // remapped so that consecutive tiles land on the same XCD (and its L2), which then
 // serves the halo re-reads.
 #include "launch.hpp"
+#include "surrogate.hpp"
 
 namespace pmf {
 
@@ -96,13 +97,29 @@ __device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
 __device__ __forceinline__ int pidx(int ix, int iy) { return (iy + 2) * FPW + (ix + 2); }
 __device__ __forceinline__ int qidx(int ix, int iy) { return (iy + 1) * FQW + (ix + 1); }
 
-template <class T, int RECON, int RS>
+// SURR: the surrogate-assisted variant (float only): each face is gated by the jump
+// detector and solved by the learned flux (wave-level MFMA) or by HLLD; simg is the
+// network's LDS image (surrogate.hpp), nsmooth counts the faces that used it (each
+// face once, as the lower face of its cell).
+template <class T, int RECON, int RS, bool SURR>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
-                                                         StepParams<T> P, double* __restrict__ red) {
+                                                         StepParams<T> P, double* __restrict__ red,
+                                                         const unsigned char* __restrict__ simg, float thr,
+                                                         unsigned long long* __restrict__ nsmooth) {
   __shared__ T sP[2][9 * FNP];    // primitive planes m (current) and m-1, alternating
   __shared__ T sLX[9 * FNQ];      // +x face states, then x-face fluxes
   __shared__ T sLY[9 * FNQ];      // +y face states, then y-face fluxes
   __shared__ T sMax[FTHREADS / 64];
+  __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
+  __shared__ float sB[SURR ? SIMG_NB : 1];
+  unsigned smooth_count = 0;
+  if constexpr (SURR) {
+    const unsigned* src = reinterpret_cast<const unsigned*>(simg);
+    unsigned* dw = reinterpret_cast<unsigned*>(sW);
+    for (int e = threadIdx.x; e < SIMG_NW / 2; e += FTHREADS) dw[e] = src[e];
+    const float* bsrc = reinterpret_cast<const float*>(simg + SIMG_NW * 2);
+    for (int e = threadIdx.x; e < SIMG_NB; e += FTHREADS) sB[e] = bsrc[e];
+  }
 
   // XCD-aware tile order: dispatch round-robins workgroups over the 8 XCDs, so
   // logical tile (w % 8) * (nwg / 8) + w / 8 gives each XCD a contiguous tile range.
@@ -156,6 +173,28 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
   }
   T smax = T(0);
 
+  // (4) the update of plane m-1 once the z face m-1/2 is known (integrator_ref.jl:140-158);
+  // shifts the z-face carry. Owned columns only.
+  auto zface_update = [&](const T* Fz, const T* Lzn, int m, bool first) {
+    if (m - 1 >= z0 && stores) {
+      T o[9];
+      const int64_t idx = (int64_t)(m - 1) * g.plane + col;
+      load_cons(U, idx, g.nc, o);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+      o[8] *= P.decay;
+#pragma unroll
+      for (int v = 0; v < 9; ++v) O.v[v][idx] = o[v];
+      smax = nanmax(smax, cell_wavespeed(o, P));
+    }
+    if (!first) {
+#pragma unroll
+      for (int v = 0; v < 9; ++v) Fzp[v] = Fz[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) Lzp[v] = Lzn[v];
+  };
+
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
@@ -180,7 +219,7 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
       for (int v = 0; v < 9; ++v) Pn[v] = q[v];
     }
     // (2) Hancock prediction of plane m (glm_physics.jl:225-231, integrator_ref.jl:17-26)
-    T Rx[9], Ry[9], Rz[9];
+    T Rx[9], Ry[9], Rz[9], Lzn[9];
     if (owned || (has_col && ring_work)) {
       T m0[9], dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
 #pragma unroll
@@ -233,25 +272,13 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
       }
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
       // whose inputs all exist now (integrator_ref.jl:140-158)
-      if (owned) {
-        T Fz[9];
-        if (!first) face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) Lzp[v] = mh[v] + dRz[v];
-        if (m - 1 >= z0 && stores) {
-          T o[9];
-          const int64_t idx = (int64_t)(m - 1) * g.plane + col;
-          load_cons(U, idx, g.nc, o);
-#pragma unroll
-          for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
-          o[8] *= P.decay;
-#pragma unroll
-          for (int v = 0; v < 9; ++v) O.v[v][idx] = o[v];
-          smax = nanmax(smax, cell_wavespeed(o, P));
-        }
-        if (!first) {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) Fzp[v] = Fz[v];
+      for (int v = 0; v < 9; ++v) Lzn[v] = mh[v] + dRz[v];
+      if constexpr (!SURR) {
+        if (owned) {
+          T Fz[9];
+          if (!first) face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+          zface_update(Fz, Lzn, m, first);
         }
       }
     }
@@ -260,11 +287,45 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
 #pragma unroll
       for (int v = 0; v < 9; ++v) nxt[v * FNP + me] = Pn[v];
     }
+    if constexpr (SURR) {
+      // waves 0-2 hold exactly the owned columns, so this branch is wave-uniform
+      if (__builtin_amdgcn_readfirstlane(t) < NOWN) {
+        T Fz[9];
+        // counts each face once: the lower faces of the stored cells
+        if (!first) smooth_count += surr_face<2>(Lzp, Rz, Fz, P, thr, true, sW, sB) && stores && !last;
+        zface_update(Fz, Lzn, m, first);
+      }
+    }
     __syncthreads();
     // (3b) x and y faces of plane m, each solved once
     const bool xy = !(first || last);
     if (xy) {
       T Fx[9], Fy[9];
+      if constexpr (SURR) {
+        T L[9], Rr[9];
+        if (xsolver) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) {
+            L[v] = sLX[v * FNQ + qidx(ix - 1, iy)];
+            Rr[v] = Rx[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) L[v] = Rr[v] = (v == 0 || v == 4) ? T(1) : T(0);
+        }
+        smooth_count += surr_face<0>(L, Rr, Fx, P, thr, xsolver, sW, sB) && stores;
+        if (ysolver) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) {
+            L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
+            Rr[v] = Ry[v];
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) L[v] = Rr[v] = (v == 0 || v == 4) ? T(1) : T(0);
+        }
+        smooth_count += surr_face<1>(L, Rr, Fy, P, thr, ysolver, sW, sB) && stores;
+      } else {
       if (xsolver) {
         T L[9];
 #pragma unroll
@@ -276,6 +337,7 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
         face_flux<1, false, T, RS>(L, Ry, Fy, P);
+      }
       }
       __syncthreads();
       if (xsolver) {
@@ -304,6 +366,11 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
     T r = sMax[0];
     for (int i = 1; i < FTHREADS / 64; ++i) r = nanmax(r, sMax[i]);
     red[blockIdx.x] = (double)r;
+  }
+  if constexpr (SURR) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) smooth_count += __shfl_xor(smooth_count, o, 64);
+    if ((t & 63) == 0 && smooth_count) atomicAdd(nsmooth, (unsigned long long)smooth_count);
   }
 }
 
@@ -362,7 +429,8 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
-#define PMF_FUSED(R, S) hipLaunchKernelGGL((k_step_fused<T, R, S>), grd, blk, 0, st, U, O, g, P, red)
+#define PMF_FUSED(R, S) \
+  hipLaunchKernelGGL((k_step_fused<T, R, S, false>), grd, blk, 0, st, U, O, g, P, red, nullptr, 0.f, nullptr)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
   else if (P.rsolve == RS_LLF) PMF_FUSED(R, RS_LLF);     \
@@ -376,6 +444,23 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
   }
 #undef PMF_FUSED_RS
 #undef PMF_FUSED
+  hipLaunchKernelGGL(k_fold_max_flat, dim3(1), dim3(256), 0, st, red, (int)nb, dsmax);
+  return true;
+}
+
+bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
+                            double* red, double* dsmax, const unsigned char* simg, float thr,
+                            unsigned long long* nsmooth, hipStream_t st) {
+  if (!fused_supported(G, 0, P.recon, P.rsolve) || P.rsolve != RS_HLLD) return false;
+  FusedGeom g = fused_geom(G, pick_zc(G));
+  unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
+  dim3 blk(FTHREADS), grd(nb);
+#define PMF_FUSED_S(R) \
+  hipLaunchKernelGGL((k_step_fused<float, R, RS_HLLD, true>), grd, blk, 0, st, U, O, g, P, red, simg, thr, nsmooth)
+  if (P.recon == RECON_PPM) PMF_FUSED_S(RECON_PPM);
+  else if (P.recon == RECON_PLM_MINMOD) PMF_FUSED_S(RECON_PLM_MINMOD);
+  else PMF_FUSED_S(RECON_PLM);
+#undef PMF_FUSED_S
   hipLaunchKernelGGL(k_fold_max_flat, dim3(1), dim3(256), 0, st, red, (int)nb, dsmax);
   return true;
 }

@@ -57,6 +57,19 @@ def lib():
             "pmf_ct_install_faces": [P, P, P, P],
             "pmf_ct_faces_from_potential": [P, P, P, P],
             "pmf_riemann_flux_batch": [P, P, P, i32, d, i64, P],
+            "pmf_surrogate_create": [P, ctypes.c_uint64, pp],
+            "pmf_surrogate_destroy": [P],
+            "pmf_surrogate_get_weights": [P, P, i64],
+            "pmf_surrogate_set_weights": [P, P, i64],
+            "pmf_surrogate_reserve": [P, i64],
+            "pmf_surrogate_dump": [P, P, d, d, d, d, ctypes.c_uint64, pi64],
+            "pmf_surrogate_dataset": [P, i64, i64, P, P, P],
+            "pmf_surrogate_upload": [P, P, P, P, i64],
+            "pmf_surrogate_grad": [P, i64, i64, i32, P, pd],
+            "pmf_surrogate_predict": [P, i64, i64, i32, P],
+            "pmf_surrogate_train": [P, i32, i32, d, ctypes.c_uint64, i32, P],
+            "pmf_surrogate_face_flux": [P, i64, i64, i32, d, P],
+            "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
             "pmf_timing_enable": [P, i32],
             "pmf_timing_read": [P, pd, pi64],
         }
@@ -64,6 +77,8 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        L.pmf_surrogate_nparams.restype = ctypes.c_int
+        L.pmf_surrogate_nparams.argtypes = []
         L.pmf_last_error.restype = ctypes.c_char_p
         L.pmf_last_error.argtypes = []
         L.pmf_version.restype = ctypes.c_char_p

@@ -308,28 +308,62 @@ def test_fused_rejects_unsupported_grids():
         M.step(b, 1e-3, ch=1.0, integrator="cube")
 
 
+class _HipArrays:
+    """Nine separately hipMalloc'ed device arrays (the runtime libpmfmhd.so uses)."""
+
+    def __init__(self, host):
+        import ctypes
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+        for a in host:
+            p = ctypes.c_void_p()
+            assert self.hip.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+            assert self.hip.hipMemcpy(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1) == 0
+            self.ptrs.append(p.value)
+        self.n = host[0].size
+
+    def get(self):
+        out = []
+        for p in self.ptrs:
+            a = np.empty(self.n, dtype=np.float32)
+            assert self.hip.hipMemcpy(a.ctypes.data_as(self.ct.c_void_p), self.ct.c_void_p(p), a.nbytes, 2) == 0
+            out.append(a)
+        return np.stack(out)
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(self.ct.c_void_p(p))
+
+
 @pytest.mark.parametrize("integrator", ["ref", "cube"])
 def test_step_into_caller_owned_arrays_is_bitwise_the_state_step(integrator):
     """The drop-in seam: nine separately allocated device arrays (as the reference's
     MHDState holds them), stepped through a field-less workspace."""
-    import torch
     dims = (32, 24, 10)
     U = smooth_state(dims, seed=21).astype(np.float32)
     s = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32)
     s.set_conserved(U)
     dt, smax = M.compute_dt(s, 0.3)
     decay = float(np.exp(np.float32(-M.GLM_CR) * smax * dt / np.float32(1.0 / dims[0])))
-    M.step_ref(s, float(dt), float(smax), decay) if integrator == "ref" else M.step_cube(s, float(dt), float(smax), decay)
+    if integrator == "ref":
+        M.step_ref(s, float(dt), float(smax), decay)
+    else:
+        M.step_cube(s, float(dt), float(smax), decay)
     want = s.conserved()
-    ins = [torch.from_numpy(U[v].copy()).to("cuda") for v in range(9)]
-    outs = [torch.empty_like(ins[v]) for v in range(9)]
-    torch.cuda.synchronize()
-    ws = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32, workspace=True)
-    assert float(M.max_wavespeed_of(ws, [t.data_ptr() for t in ins])) == float(smax)
-    M.step_into(ws, [t.data_ptr() for t in ins], [t.data_ptr() for t in outs], float(dt), float(smax), decay,
-                integrator=integrator)
-    ws.synchronize()
-    got = np.stack([t.cpu().numpy() for t in outs])
-    assert np.array_equal(got, want)
-    with pytest.raises(M.MHDError, match="workspace without fields"):
-        M.compute_dt(ws)
+    ins = _HipArrays([U[v].copy() for v in range(9)])
+    outs = _HipArrays([np.zeros_like(U[v]) for v in range(9)])
+    try:
+        ws = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32, workspace=True)
+        assert float(M.max_wavespeed_of(ws, ins.ptrs)) == float(smax)
+        M.step_into(ws, ins.ptrs, outs.ptrs, float(dt), float(smax), decay, integrator=integrator)
+        ws.synchronize()
+        assert np.array_equal(outs.get(), want)
+        with pytest.raises(M.MHDError, match="workspace without fields"):
+            M.compute_dt(ws)
+    finally:
+        ins.free()
+        outs.free()
