@@ -164,6 +164,32 @@ int pmf_surrogate_face_flux(pmf_surrogate* n, int64_t first, int64_t count, int3
 int pmf_step_surrogate(pmf_state* s, pmf_surrogate* n, double dt, double ch, double glm_cr, double thr,
                        int64_t* nsmooth);
 
+/* ---- Ensembles ------------------------------------------------------------------
+ * No counterpart in the reference, which advances one MHDState per step! call
+ * (driver.jl:20-37): many independent periodic float32 GLM cubes of uneven sizes
+ * (edges multiples of 16, dx = 1/edge) advanced together, one launch per stage (the
+ * per-cube CFL step of evolve!, driver.jl:48-71, computed on the device; a persistent
+ * step over all cubes' tiles), the step pair captured in a HIP graph. base supplies
+ * gamma, the floors, llf_dmin/llf_pmin, recon and riemann (nx, ny, nz, dx, bc ignored). */
+typedef struct pmf_ensemble pmf_ensemble;
+int pmf_ensemble_create(const pmf_params* base, int32_t ncubes, const int32_t* edge, pmf_ensemble** out);
+int pmf_ensemble_destroy(pmf_ensemble* e);
+/* Conserved variable var of the cube's current state (edge^3 float32 values, i fastest). */
+int pmf_ensemble_upload(pmf_ensemble* e, int32_t cube, int32_t var, const void* host, int64_t n);
+int pmf_ensemble_download(pmf_ensemble* e, int32_t cube, int32_t var, void* host, int64_t n);
+/* nsteps steps of every cube, each with its own dt = cfl dx / smax, ch = glm_ch_fac smax,
+ * dt_used = dt smax / max(smax, ch) and psi decay exp(-glm_cr ch dt_used / dx). use_graph
+ * replays a captured two-step graph. seconds (optional) = GPU time of the steps. */
+int pmf_ensemble_run(pmf_ensemble* e, int32_t nsteps, double cfl, double glm_ch_fac, double glm_cr, int32_t use_graph,
+                     double* seconds);
+/* Per-cube elapsed time, last dt and the CFL speed the last dt came from (arrays of ncubes). */
+int pmf_ensemble_clock(pmf_ensemble* e, double* t, double* dt, double* smax);
+/* Autotuner: chooses the tile z-chunk length per cube edge and the persistent grid size
+ * by timing reps steps per candidate (the state is not advanced). */
+int pmf_ensemble_tune(pmf_ensemble* e, int32_t reps, double* ms_per_step);
+int pmf_ensemble_config(pmf_ensemble* e, int32_t* zc, int32_t* nwg, int32_t* nitems);
+int pmf_ensemble_set_config(pmf_ensemble* e, const int32_t* zc, int32_t nwg);
+
 const char* pmf_last_error(void);
 const char* pmf_version(void);
 

@@ -30,6 +30,38 @@ bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& 
 // Number of workgroups (one partial maximum each) the fused kernel launches for G.
 int64_t fused_blocks(const Grid& G);
 
+// Geometry of the fused kernel's work decomposition: 16x12 column tiles (x fastest,
+// then y; the last tile row may be ragged) marched through z chunks of zc planes.
+struct FusedGeom {
+  int32_t nx, ny, nz, ntx, nty, nchunks, zc;
+  int64_t plane, nc;
+};
+constexpr int kFusedTX = 16, kFusedTY = 12;
+
+// One cube of an ensemble: both field sets (the step reads A and writes B on even
+// steps, the reverse on odd ones), its geometry and its slice of the work-item list.
+struct EnsCube {
+  Fields<float> A, B;
+  FusedGeom g;
+  double dx;
+  int32_t item0, nitems;
+};
+// Eight work queues (one per XCD): contiguous slices [base, base + len) of the items.
+struct EnsQueues {
+  int32_t base[8], len[8];
+};
+FusedGeom ens_geom(int n, int zc);
+// Initial CFL maxima, one per work item (items are (cube, tile) pairs).
+void launch_ens_wavespeed(const EnsCube* cubes, const int2* items, int nitems, const StepParams<float>& P,
+                          double* red, int parity, hipStream_t st);
+// Per-cube dt / ch / decay from the item maxima in red; clock holds (t, dt, smax) per cube.
+void launch_ens_params(const EnsCube* cubes, int ncubes, const double* red, const StepParams<float>& base,
+                       StepParams<float>* params, double* clock, unsigned* ctr, float cfl, float fac, float cr,
+                       hipStream_t st);
+// The persistent step over all items (nwg workgroups; ctr = eight queue counters).
+void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q, const StepParams<float>* params,
+                     double* red, unsigned* ctr, int nwg, int parity, int recon, int rsolve, hipStream_t st);
+
 template <class T>
 void launch_wavespeed(Fields<const T> U, const Grid& G, const StepParams<T>& P, double* red, double* out,
                       hipStream_t st);

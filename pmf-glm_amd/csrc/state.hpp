@@ -140,8 +140,10 @@ inline void fetch(pmf_state* s, int first, int count) {
 // GLM psi damping factor exp(-cr ch dt / dx), evaluated in the state's precision.
 inline double decay_of(const pmf_state* s, double dt, double ch, double glm_cr) {
   if (s->p.precision == PMF_F64) return std::exp(-glm_cr * ch * dt / s->p.dx);
+  // float32: exp evaluated in float64 and rounded once (the ensemble's device-side CFL
+  // stage does the same, so both paths get the same factor)
   float v = -float(glm_cr) * float(ch) * float(dt) / float(s->p.dx);
-  return double(std::exp(v));
+  return double(float(std::exp(double(v))));
 }
 
 }  // namespace pmf

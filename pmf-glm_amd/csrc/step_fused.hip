@@ -25,7 +25,7 @@
 
 namespace pmf {
 
-constexpr int TX = 16, TY = 12;       // owned tile (x, y)
+constexpr int TX = kFusedTX, TY = kFusedTY;  // owned tile (x, y)
 constexpr int NOWN = TX * TY;          // 192
 constexpr int NRING = 2 * TX + 2 * TY; // 56 (one-cell ring without corners)
 constexpr int FPW = TX + 4;            // LDS prim plane width (two halo rings)
@@ -35,11 +35,6 @@ constexpr int FNQ = FQW * (TY + 2);    // 252
 constexpr int FNFX = (TX + 1) * TY;    // x faces of a tile plane: 204
 constexpr int FNFY = TX * (TY + 1);    // y faces of a tile plane: 208
 constexpr int FTHREADS = 256;
-
-struct FusedGeom {
-  int32_t nx, ny, nz, ntx, nty, nchunks, zc;
-  int64_t plane, nc;
-};
 
 __device__ __forceinline__ int wrapi(int i, int n) {
   int r = i % n;
@@ -97,35 +92,26 @@ __device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
 __device__ __forceinline__ int pidx(int ix, int iy) { return (iy + 2) * FPW + (ix + 2); }
 __device__ __forceinline__ int qidx(int ix, int iy) { return (iy + 1) * FQW + (ix + 1); }
 
-// SURR: the surrogate-assisted variant (float only): each face is gated by the jump
-// detector and solved by the learned flux (wave-level MFMA) or by HLLD; simg is the
-// network's LDS image (surrogate.hpp), nsmooth counts the faces that used it (each
-// face once, as the lower face of its cell).
-template <class T, int RECON, int RS, bool SURR>
-__global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
-                                                         StepParams<T> P, double* __restrict__ red,
-                                                         const unsigned char* __restrict__ simg, float thr,
-                                                         unsigned long long* __restrict__ nsmooth) {
-  __shared__ T sP[2][9 * FNP];    // primitive planes m (current) and m-1, alternating
-  __shared__ T sLX[9 * FNQ];      // +x face states, then x-face fluxes
-  __shared__ T sLY[9 * FNQ];      // +y face states, then y-face fluxes
-  __shared__ T sMax[FTHREADS / 64];
-  __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
-  __shared__ float sB[SURR ? SIMG_NB : 1];
-  unsigned smooth_count = 0;
-  if constexpr (SURR) {
-    const unsigned* src = reinterpret_cast<const unsigned*>(simg);
-    unsigned* dw = reinterpret_cast<unsigned*>(sW);
-    for (int e = threadIdx.x; e < SIMG_NW / 2; e += FTHREADS) dw[e] = src[e];
-    const float* bsrc = reinterpret_cast<const float*>(simg + SIMG_NW * 2);
-    for (int e = threadIdx.x; e < SIMG_NB; e += FTHREADS) sB[e] = bsrc[e];
-  }
+// LDS of one workgroup of the fused step.
+template <class T>
+struct FusedLds {
+  T sP[2][9 * FNP];  // primitive planes m (current) and m-1, alternating
+  T sLX[9 * FNQ];    // +x face states, then x-face fluxes
+  T sLY[9 * FNQ];    // +y face states, then y-face fluxes
+};
 
-  // XCD-aware tile order: dispatch round-robins workgroups over the 8 XCDs, so
-  // logical tile (w % 8) * (nwg / 8) + w / 8 gives each XCD a contiguous tile range.
-  const int nwg = gridDim.x;
-  int w = blockIdx.x;
-  if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+// One work item: tile w (x fastest, then y, then z chunk) of the grid g, marched through
+// its z chunk. Called by all FTHREADS threads of the workgroup; returns the calling
+// thread's running CFL maximum of the cells it stored. SURR: the surrogate-assisted
+// variant (float only): each face is gated by the jump detector and solved by the
+// learned flux (wave-level MFMA) or by HLLD; sW/sB hold the network's LDS image
+// (surrogate.hpp) and smooth_count counts the faces that used it (each face once, as
+// the lower face of its cell).
+template <class T, int RECON, int RS, bool SURR>
+__device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
+                                        const StepParams<T>& P, int w, FusedLds<T>& S,
+                                        const unsigned short* __restrict__ sW, const float* __restrict__ sB,
+                                        float thr, unsigned& smooth_count) {
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -158,11 +144,11 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
       load_cons(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, g.nc, c);
       cons2prim(c, q, P);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) sP[(z0 - 2) & 1][v * FNP + me] = q[v];
+      for (int v = 0; v < 9; ++v) S.sP[(z0 - 2) & 1][v * FNP + me] = q[v];
       load_cons(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, g.nc, c);
       cons2prim(c, q, P);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) sP[(z0 - 1) & 1][v * FNP + me] = q[v];
+      for (int v = 0; v < 9; ++v) S.sP[(z0 - 1) & 1][v * FNP + me] = q[v];
     }
   }
 #pragma unroll
@@ -198,8 +184,8 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
-    const T* __restrict__ cur = sP[m & 1];
-    T* __restrict__ nxt = sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
+    const T* __restrict__ cur = S.sP[m & 1];
+    T* __restrict__ nxt = S.sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
     // (1) plane m+1 of the own column and of the extra cell (in flight during the barrier)
     T Pn[9], Xc[9];
     if (has_col) load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, g.nc, Pn);
@@ -264,8 +250,8 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
       // upper face, the -d edge the right state of the lower face
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
-        sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
-        sLY[v * FNQ + qidx(ix, iy)] = mh[v] + dRy[v];
+        S.sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
+        S.sLY[v * FNQ + qidx(ix, iy)] = mh[v] + dRy[v];
         Rx[v] = mh[v] + dLx[v];
         Ry[v] = mh[v] + dLy[v];
         Rz[v] = mh[v] + dLz[v];
@@ -306,7 +292,7 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
         if (xsolver) {
 #pragma unroll
           for (int v = 0; v < 9; ++v) {
-            L[v] = sLX[v * FNQ + qidx(ix - 1, iy)];
+            L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
             Rr[v] = Rx[v];
           }
         } else {
@@ -317,7 +303,7 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
         if (ysolver) {
 #pragma unroll
           for (int v = 0; v < 9; ++v) {
-            L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
+            L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
             Rr[v] = Ry[v];
           }
         } else {
@@ -329,35 +315,41 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
       if (xsolver) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = sLX[v * FNQ + qidx(ix - 1, iy)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
         face_flux<0, false, T, RS>(L, Rx, Fx, P);
       }
       if (ysolver) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = sLY[v * FNQ + qidx(ix, iy - 1)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
         face_flux<1, false, T, RS>(L, Ry, Fy, P);
       }
       }
       __syncthreads();
       if (xsolver) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
+        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
       }
       if (ysolver) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) sLY[v * FNFY + iy * TX + ix] = Fy[v];
+        for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + iy * TX + ix] = Fy[v];
       }
       __syncthreads();
       if (owned) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
-          Dxy[v] = (sLX[v * FNFX + iy * (TX + 1) + ix] - sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
-                   (sLY[v * FNFY + iy * TX + ix] - sLY[v * FNFY + (iy + 1) * TX + ix]);
+          Dxy[v] = (S.sLX[v * FNFX + iy * (TX + 1) + ix] - S.sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
+                   (S.sLY[v * FNFY + iy * TX + ix] - S.sLY[v * FNFY + (iy + 1) * TX + ix]);
       }
     }
   }
-  // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
+  return smax;
+}
+
+// Block-wide CFL maximum of the threads' running maxima, stored by thread 0.
+template <class T>
+__device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
+  const int t = threadIdx.x;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) smax = nanmax(smax, __shfl_xor(smax, o, 64));
   if ((t & 63) == 0) sMax[t >> 6] = smax;
@@ -365,12 +357,41 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
   if (t == 0) {
     T r = sMax[0];
     for (int i = 1; i < FTHREADS / 64; ++i) r = nanmax(r, sMax[i]);
-    red[blockIdx.x] = (double)r;
+    *out = (double)r;
   }
+}
+
+// The single-grid step: one workgroup per work item. simg is the network's image in
+// global memory (SURR only) and nsmooth the count of network-solved faces.
+template <class T, int RECON, int RS, bool SURR>
+__global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
+                                                         StepParams<T> P, double* __restrict__ red,
+                                                         const unsigned char* __restrict__ simg, float thr,
+                                                         unsigned long long* __restrict__ nsmooth) {
+  __shared__ FusedLds<T> lds;
+  __shared__ T sMax[FTHREADS / 64];
+  __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
+  __shared__ float sB[SURR ? SIMG_NB : 1];
+  unsigned smooth_count = 0;
+  if constexpr (SURR) {
+    const unsigned* src = reinterpret_cast<const unsigned*>(simg);
+    unsigned* dw = reinterpret_cast<unsigned*>(sW);
+    for (int e = threadIdx.x; e < SIMG_NW / 2; e += FTHREADS) dw[e] = src[e];
+    const float* bsrc = reinterpret_cast<const float*>(simg + SIMG_NW * 2);
+    for (int e = threadIdx.x; e < SIMG_NB; e += FTHREADS) sB[e] = bsrc[e];
+  }
+  // XCD-aware tile order: dispatch round-robins workgroups over the 8 XCDs, so
+  // logical tile (w % 8) * (nwg / 8) + w / 8 gives each XCD a contiguous tile range.
+  const int nwg = gridDim.x;
+  int w = blockIdx.x;
+  if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+  const T smax = fused_item<T, RECON, RS, SURR>(U, O, g, P, w, lds, sW, sB, thr, smooth_count);
+  // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
+  block_store_max(smax, sMax, red + blockIdx.x);
   if constexpr (SURR) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) smooth_count += __shfl_xor(smooth_count, o, 64);
-    if ((t & 63) == 0 && smooth_count) atomicAdd(nsmooth, (unsigned long long)smooth_count);
+    if ((threadIdx.x & 63) == 0 && smooth_count) atomicAdd(nsmooth, (unsigned long long)smooth_count);
   }
 }
 
@@ -463,6 +484,166 @@ bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& 
 #undef PMF_FUSED_S
   hipLaunchKernelGGL(k_fold_max_flat, dim3(1), dim3(256), 0, st, red, (int)nb, dsmax);
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble: many independent periodic cubes advanced together. Each stage is one
+// launch over all cubes: the step is a persistent kernel whose workgroups draw work
+// items (cube, tile) from eight queues, one per XCD, each a contiguous cost-balanced
+// slice of the item list (so neighbouring tiles share an L2), stealing from the other
+// queues once their own is empty; the per-cube time step comes from a device-side
+// stage that folds the step's per-item CFL maxima. No host round trip per step, so
+// the step pair can be captured in a graph.
+
+__global__ __launch_bounds__(FTHREADS) void k_ens_wavespeed(const EnsCube* __restrict__ cubes,
+                                                          const int2* __restrict__ items, StepParams<float> P,
+                                                          double* __restrict__ red, int parity) {
+  __shared__ float sMax[FTHREADS / 64];
+  const int2 it = items[blockIdx.x];
+  const int c = __builtin_amdgcn_readfirstlane(it.x), w = __builtin_amdgcn_readfirstlane(it.y);
+  const EnsCube& cb = cubes[c];
+  const FusedGeom& g = cb.g;
+  const Fields<float>& U = parity ? cb.B : cb.A;
+  const int tx = w % g.ntx, ty = (w / g.ntx) % g.nty, chunk = w / (g.ntx * g.nty);
+  const int z0 = chunk * g.zc, zc = min(g.zc, g.nz - z0);
+  const int t = threadIdx.x;
+  float smax = 0.f;
+  if (t < NOWN) {
+    const int ix = tx * TX + t % TX, iy = ty * TY + t / TX;
+    if (iy < g.ny) {
+      for (int z = z0; z < z0 + zc; ++z) {
+        const int64_t idx = (int64_t)z * g.plane + (int64_t)iy * g.nx + ix;
+        float q[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) q[v] = U.v[v][idx];
+        smax = nanmax(smax, cell_wavespeed(q, P));
+      }
+    }
+  }
+  block_store_max(smax, sMax, red + blockIdx.x);
+}
+
+// Per-cube CFL stage (timestep.jl:39 and driver.jl:48-71 in float32 arithmetic, the same
+// operations as the single-state host path): fold the cube's item maxima, then
+// dt = cfl dx / max(smax, eps), ch = glm_ch_fac smax, dt_used = dt smax / max(smax, ch),
+// decay = exp(-cr ch dt_used / dx). Also clears the step's work queues.
+__global__ __launch_bounds__(256) void k_ens_params(const EnsCube* __restrict__ cubes, const double* __restrict__ red,
+                                                    StepParams<float> base, StepParams<float>* __restrict__ params,
+                                                    double* __restrict__ clock, unsigned* __restrict__ ctr, float cfl,
+                                                    float fac, float cr) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const EnsCube& cb = cubes[c];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < cb.nitems; i += 256) m = nanmax(m, red[cb.item0 + i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double smax = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
+    const float smf = float(smax), dxf = float(cb.dx);
+    const double dt = double(cfl * dxf / fmaxf(smf, __FLT_EPSILON__));
+    const float chf = fac * smf;
+    const double dtf = dt * double(smf) / double(fmaxf(smf, chf));
+    StepParams<float> P = base;
+    P.ch = chf;
+    P.ich = chf != 0.f ? 1.f / chf : 0.f;
+    P.ch2 = chf * chf;
+    P.dtdx = float(dtf) / dxf;
+    P.hdtdx = 0.5f * P.dtdx;
+    const float v = -cr * chf * float(dtf) / dxf;
+    P.decay = float(exp(double(v)));
+    params[c] = P;
+    clock[3 * c] += dtf;
+    clock[3 * c + 1] = dtf;
+    clock[3 * c + 2] = smax;
+    if (c == 0) {
+      for (int q = 0; q < 8; ++q) ctr[q] = 0u;
+    }
+  }
+}
+
+template <int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) void k_ens_step(const EnsCube* __restrict__ cubes,
+                                                     const int2* __restrict__ items, EnsQueues q,
+                                                     const StepParams<float>* __restrict__ params,
+                                                     double* __restrict__ red, unsigned* __restrict__ ctr,
+                                                     int parity) {
+  __shared__ FusedLds<float> lds;
+  __shared__ float sMax[FTHREADS / 64];
+  __shared__ int sItem;
+  unsigned none = 0;
+  int qi = 0;  // queues tried so far (thread 0)
+  const int home = blockIdx.x & 7;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int got = -1;
+      while (qi < 8) {
+        const int qq = (home + qi) & 7;
+        const unsigned k = atomicAdd(&ctr[qq], 1u);
+        if (k < (unsigned)q.len[qq]) {
+          got = q.base[qq] + (int)k;
+          break;
+        }
+        ++qi;
+      }
+      sItem = got;
+    }
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(sItem);
+    if (i < 0) break;  // uniform: every queue is drained
+    const int2 it = items[i];
+    const int c = __builtin_amdgcn_readfirstlane(it.x), w = __builtin_amdgcn_readfirstlane(it.y);
+    const EnsCube& cb = cubes[c];
+    const Fields<float>& in = parity ? cb.B : cb.A;
+    const Fields<float>& out = parity ? cb.A : cb.B;
+    Fields<const float> U;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) U.v[v] = in.v[v];
+    const StepParams<float> P = params[c];
+    const float smax = fused_item<float, RECON, RS, false>(U, out, cb.g, P, w, lds, nullptr, nullptr, 0.f, none);
+    block_store_max(smax, sMax, red + i);
+    __syncthreads();  // sItem and sMax are reused by the next item
+  }
+}
+
+FusedGeom ens_geom(int n, int zc) {
+  Grid G;
+  G.nx = G.ny = G.nz = n;
+  G.nc = (int64_t)n * n * n;
+  return fused_geom(G, zc);
+}
+
+void launch_ens_wavespeed(const EnsCube* cubes, const int2* items, int nitems, const StepParams<float>& P,
+                          double* red, int parity, hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_wavespeed, dim3(nitems), dim3(FTHREADS), 0, st, cubes, items, P, red, parity);
+}
+
+void launch_ens_params(const EnsCube* cubes, int ncubes, const double* red, const StepParams<float>& base,
+                       StepParams<float>* params, double* clock, unsigned* ctr, float cfl, float fac, float cr,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_params, dim3(ncubes), dim3(256), 0, st, cubes, red, base, params, clock, ctr, cfl, fac,
+                     cr);
+}
+
+void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q, const StepParams<float>* params,
+                     double* red, unsigned* ctr, int nwg, int parity, int recon, int rsolve, hipStream_t st) {
+#define PMF_ENS(R, S) \
+  hipLaunchKernelGGL((k_ens_step<R, S>), dim3(nwg), dim3(FTHREADS), 0, st, cubes, items, q, params, red, ctr, parity)
+#define PMF_ENS_RS(R)                                 \
+  if (rsolve == RS_HLL) PMF_ENS(R, RS_HLL);           \
+  else if (rsolve == RS_LLF) PMF_ENS(R, RS_LLF);      \
+  else PMF_ENS(R, RS_HLLD);
+  if (recon == RECON_PPM) {
+    PMF_ENS_RS(RECON_PPM)
+  } else if (recon == RECON_PLM_MINMOD) {
+    PMF_ENS_RS(RECON_PLM_MINMOD)
+  } else {
+    PMF_ENS_RS(RECON_PLM)
+  }
+#undef PMF_ENS_RS
+#undef PMF_ENS
 }
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,

@@ -70,6 +70,15 @@ def lib():
             "pmf_surrogate_train": [P, i32, i32, d, ctypes.c_uint64, i32, P],
             "pmf_surrogate_face_flux": [P, i64, i64, i32, d, P],
             "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
+            "pmf_ensemble_create": [ctypes.POINTER(pmf_params), i32, P, ctypes.POINTER(ctypes.c_void_p)],
+            "pmf_ensemble_destroy": [P],
+            "pmf_ensemble_upload": [P, i32, i32, P, i64],
+            "pmf_ensemble_download": [P, i32, i32, P, i64],
+            "pmf_ensemble_run": [P, i32, d, d, d, i32, pd],
+            "pmf_ensemble_clock": [P, P, P, P],
+            "pmf_ensemble_tune": [P, i32, pd],
+            "pmf_ensemble_config": [P, P, P, P],
+            "pmf_ensemble_set_config": [P, P, i32],
             "pmf_timing_enable": [P, i32],
             "pmf_timing_read": [P, pd, pi64],
         }

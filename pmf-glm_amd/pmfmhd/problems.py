@@ -271,3 +271,28 @@ def config_state(name, seed=1, n=None):
         kw["kmax"] = max(1, int(round(kw["kmax"] * scale)))
         kw["n"] = n
     return random_mhd(seed=seed, **kw)
+
+
+
+# The ensemble workload: 32 cubes of 32^3, 24 of 64^3 and 8 of 128^3; cube c has seed c,
+# rms |B| uniform in [0.1, 2.0] and spectral index n_B uniform in [-2.5, 2.0] (both drawn
+# from the splitmix stream of the cube id), cutoff |m| <= N/4, and otherwise the
+# 32^3 benchmark recipe (rho = 1 + 0.01 N(0,1), p = 1, v = 0).
+ENSEMBLE_LAYOUT = ((32, 32), (24, 64), (8, 128))
+
+
+def ensemble_members(layout=ENSEMBLE_LAYOUT):
+    out = []
+    c = 0
+    for count, n in layout:
+        for _ in range(count):
+            u = hash_uniform(c, 21, np.arange(2, dtype=np.int64))
+            out.append(dict(cube=c, n=n, seed=c, brms=0.1 + 1.9 * float(u[0]), b_index=-2.5 + 4.5 * float(u[1])))
+            c += 1
+    return out
+
+
+def ensemble_state(member):
+    m = member
+    return random_mhd(m["n"], seed=m["seed"], kmax=m["n"] // 4, brms=m["brms"], b_index=m["b_index"],
+                      vrms_mach=0.0, rho_mode="gauss", rho_sigma=0.01)

@@ -76,7 +76,7 @@ def test_dump_faces_match_oracle_riemann_and_training_beats_central(gpu_state):
             dt, smax = M.compute_dt(s, 0.2)
             total = net.dump(s, float(dt), float(smax), keep=0.5, seed=100 * seed + it)
             M.step(s, float(dt), ch=float(smax))
-    assert total > 100000
+    assert total > 40000
     X, Y, S = net.dataset(0, 5000)
     assert np.isfinite(X).all() and np.isfinite(Y).all()
     # the dumped faces' HLLD flux is the oracle's riemann on the same normal-frame states
@@ -193,7 +193,7 @@ def test_dump_inputs_and_targets_match_oracle_encoding(gpu_state):
     s.set_conserved(P.config_state("c1_32", seed=8))
     dt, smax = M.compute_dt(s, 0.2)
     n = net.dump(s, float(dt), float(smax), keep=1.0, seed=1)
-    assert n > 10000
+    assert n > 5000
     X, Y, S = net.dataset(0, min(n, 20000))
     L = np.concatenate([S[:, :8].T, np.zeros((1, len(S)))]).astype(np.float64)
     Rr = np.concatenate([S[:, 8:16].T, np.zeros((1, len(S)))]).astype(np.float64)
