@@ -1,4 +1,6 @@
-"""Throughput of the plain GLM-HLLD step on one MI355X.
+"""Throughput of the plain GLM-HLLD step on one MI355X (the JSON line's value), with the
+float32-vs-float64 error report, the surrogate-assisted step against the plain step,
+and the batched ensemble as extra objects of the same line.
 
 Workload: the 256^3 float32 benchmark configuration (PLM minmod, HLLD, CFL 0.2,
 gamma 5/3, periodic unit box; seeded solenoidal Gaussian B with P(k)~k^2, |m|<=32,
@@ -46,6 +48,125 @@ def cpu_baseline(U, n, steps):
                        f"C++ restatement of the reference :ref kernel, {el:.1f} s")
 
 
+VARS = ("rho", "mx", "my", "mz", "E", "bx", "by", "bz")
+
+
+def rel_l2(U, Uref):
+    """Per-variable relative L2 of U against Uref (9, nc), and psi in absolute terms."""
+    import numpy as np
+    out = {}
+    for v, name in enumerate(VARS):
+        d = np.linalg.norm(U[v].astype(np.float64) - Uref[v])
+        out[name] = float(d / max(np.linalg.norm(Uref[v]), 1e-300))
+    num = np.linalg.norm((U[:8].astype(np.float64) - Uref[:8]).ravel())
+    out["all_but_psi"] = float(num / np.linalg.norm(Uref[:8].ravel()))
+    out["psi_rms_abs"] = float(np.sqrt(np.mean((U[8].astype(np.float64) - Uref[8]) ** 2)))
+    return out
+
+
+def fp64_sequence(M, U, n, steps, cfl=0.2):
+    """Run the float64 GPU path (equal to the float64 CPU oracle to 1e-12 in the parity
+    tests) for `steps` CFL steps; returns the final state and the (dt, ch) sequence."""
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype="float64", recon="plm_minmod")
+    s.set_conserved(U)
+    seq = []
+    for _ in range(steps):
+        dt, sm = M.compute_dt(s, cfl)
+        seq.append((float(dt), float(sm)))
+        M.step(s, float(dt), ch=float(sm), integrator="cube")
+    out = s.conserved()
+    del s
+    return out, seq
+
+
+def replay(M, U, n, seq, surrogate=None, thr=None):
+    """Float32 replay of a (dt, ch) sequence, plain or surrogate-assisted; returns the
+    final state, wall seconds of the step loop and the network-solved face counts of the
+    first and last step."""
+    import numpy as np
+    from pmfmhd.surrogate import Surrogate, step_surrogate
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(U)
+    net = None
+    if surrogate is not None:
+        net = Surrogate(s, seed=0)
+        net.set_weights(surrogate)
+    counts = []
+    s.synchronize()
+    t0 = time.perf_counter()
+    for k, (dt, ch) in enumerate(seq):
+        if net is None:
+            M.step(s, dt, ch=ch, integrator="cube")
+        else:
+            c = step_surrogate(s, net, dt, ch, glm_cr=M.GLM_CR, thr=thr, count=(k == 0 or k == len(seq) - 1))
+            if c is not None:
+                counts.append(c)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    return s.conserved(), el, counts
+
+
+def error_report(M, P, n, steps):
+    """Float32 plain step against float64 on the same (dt, ch) sequence (oracle-dt replay)."""
+    U = P.config_state("c2_256", seed=1, n=n)
+    ref, seq = fp64_sequence(M, U, n, steps)
+    got, _, _ = replay(M, U, n, seq)
+    return {"workload": f"plain GLM-HLLD {n}^3, seed 1, {steps} CFL-0.2 steps", "steps": steps,
+            "reference": "float64 GPU fused kernel (float64 oracle parity 1e-12), dt sequence replayed in float32",
+            "rel_l2_fp32_vs_fp64": rel_l2(got, ref)}
+
+
+def surrogate_leg(M, P, n, steps, weights, thr):
+    """The surrogate-assisted step against the plain step on the 32^3-recipe inputs at n^3
+    (rms |B| 0.3, |m| <= 8, v = 0): throughput of both, and each one's whole-run error
+    against float64 on the same (dt, ch) sequence."""
+    import numpy as np
+    U = P.config_state("c4_256", seed=1, n=n)
+    ref, seq = fp64_sequence(M, U, n, steps)
+    plain, t_plain, _ = replay(M, U, n, seq)
+    sur, t_sur, counts = replay(M, U, n, seq, surrogate=weights, thr=thr)
+    cells = n ** 3
+    e_plain, e_sur = rel_l2(plain, ref), rel_l2(sur, ref)
+    return {"workload": f"surrogate-assisted GLM-HLLD FP32 {n}^3 on the 32^3-recipe inputs (|m|<=8, rms|B| 0.3, "
+                        f"v=0), seed 1, {steps} steps, gate thr {thr}",
+            "value": cells * steps / t_sur, "plain_value": cells * steps / t_plain,
+            "speedup_vs_plain": t_plain / t_sur,
+            "network_face_fraction_first_last": [c / (3 * cells) for c in counts],
+            "rel_l2_vs_fp64": {"plain": e_plain, "surrogate": e_sur},
+            "meets_rel_l2_1e-4": bool(e_sur["all_but_psi"] <= 1e-4)}
+
+
+def ensemble_leg(P):
+    """The 64-cube ensemble (32 x 32^3, 24 x 64^3, 8 x 128^3), autotuned, 100 steps."""
+    import numpy as np
+    from pmfmhd.ensemble import Ensemble
+    mem = P.ensemble_members()
+    e = Ensemble([m["n"] for m in mem])
+    for c, m in enumerate(mem):
+        e.upload(c, P.ensemble_state(m))
+    ms = e.tune(reps=3)
+    zc, nwg, nitems = e.config()
+    sec = e.run(100)
+    cells = sum(m["n"] ** 3 for m in mem)
+    return {"workload": "ensemble: 32 x 32^3 + 24 x 64^3 + 8 x 128^3 FP32 GLM-HLLD, own CFL dt per cube, 100 steps, "
+                        "one persistent launch per stage, two-step HIP graph",
+            "cubes": len(mem), "cells": cells, "value": cells * 100 / sec, "unit": "cell-updates/s",
+            "ms_per_step": sec * 10, "tuned_ms_per_step": ms,
+            "tuned": {"zc_by_edge": {str(m["n"]): int(z) for m, z in zip(mem, zc)}, "workgroups": nwg,
+                      "work_items": nitems}}
+
+
+def traffic_from_profile(n, integ):
+    """HBM bytes per fused launch from the committed FETCH_SIZE / WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", "k_step_fused_256cubed_traffic.md")
+    if n != 256 or integ != "cube" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +175,13 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--integrator", default="auto", choices=["auto", "cube", "ref"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--error-steps", type=int, default=200,
+                    help="steps of the float32-vs-float64 error report (0 = skip)")
+    ap.add_argument("--surrogate-steps", type=int, default=200,
+                    help="steps of the surrogate-vs-plain leg (0 = skip)")
+    ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "pmf-glm_amd", "pmfmhd", "data",
+                                                                  "surrogate_weights.npy"))
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
                          "size that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -127,6 +255,19 @@ def main():
         log(f"bench: CPU baseline ({args.cpu_steps} step(s))")
         cpu = cpu_baseline(U, n, args.cpu_steps)
 
+    del s
+    errors = sur = ens = None
+    if args.error_steps > 0 and integ == "cube":
+        log(f"bench: float32 vs float64 error report ({args.error_steps} steps)")
+        errors = error_report(M, P, n, args.error_steps)
+    if args.surrogate_steps > 0 and integ == "cube" and os.path.exists(args.surrogate_weights):
+        from pmfmhd.surrogate import THR
+        log(f"bench: surrogate leg ({args.surrogate_steps} steps)")
+        sur = surrogate_leg(M, P, n, args.surrogate_steps, np.load(args.surrogate_weights), THR)
+    if not args.no_ensemble:
+        log("bench: ensemble leg")
+        ens = ensemble_leg(P)
+
     kernel = "k_step_fused" if integ == "cube" else "k_step_ref"
     out = {
         "metric": METRIC,
@@ -146,9 +287,12 @@ def main():
                    "n": n, "recon": "plm_minmod", "riemann": "hlld", "cfl": 0.2, "integrator": integ,
                    "parallelism": "single GPU", "finite": bool(finite)},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n, integ),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_launch": BYTES_PER_CELL * cells},
         "cpu_baseline": cpu,
+        "error_report": errors,
+        "surrogate": sur,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

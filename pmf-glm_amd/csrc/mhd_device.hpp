@@ -130,6 +130,18 @@ template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0
     T eR = flat ? q0 : ((diff <= lim && diff < -lim) ? T(3) * q0 - T(2) * qL : qR);
     dL = eL - q0;
     dR = eR - q0;
+  } else if constexpr (sizeof(T) == 4) {
+    // minmod(a, b) is the median of (a, b, 0), and MonCen the median of (2 minmod(a, b),
+    // (a+b)/2, 0): one v_med3_f32 each instead of the sign/abs/min/compare chain (equal
+    // to glm_physics.jl's form except where a*b underflows to zero)
+    T dl = q0 - qm, dr = qp - q0;
+    T mm = __builtin_amdgcn_fmed3f(dl, dr, T(0));
+    T s;
+    if constexpr (RECON == RECON_PLM_MINMOD) s = mm;
+    else s = __builtin_amdgcn_fmed3f(T(2) * mm, T(0.5) * (dl + dr), T(0));
+    s = T(0.5) * s;
+    dL = -s;
+    dR = s;
   } else {
     T dl = q0 - qm, dr = qp - q0;
     T s;
@@ -204,8 +216,6 @@ __device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fp
   T irL = rcp(dL), irR = rcp(dR);
   T b2L = bn2 + byL * byL + bzL * bzL, b2R = bn2 + byR * byR + bzR * bzR;
   T ptL = pL + T(0.5) * b2L, ptR = pR + T(0.5) * b2R;
-  T EL = pL * P.igm1 + T(0.5) * dL * (uL * uL + vL * vL + wL * wL) + T(0.5) * b2L;
-  T ER = pR * P.igm1 + T(0.5) * dR * (uR * uR + vR * vR + wR * wR) + T(0.5) * b2R;
   T cfL = cfast(irL, pL, b2L, bn, P.gamma), cfR = cfast(irR, pR, b2R, bn, P.gamma);
   T cmax = dmax(cfL, cfR);
   T SL = dmin(dmin(uL, uR) - cmax, T(0));
@@ -244,11 +254,7 @@ __device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fp
     byRs = deg ? byR : byR * gg;
     bzRs = deg ? bzR : bzR * gg;
   }
-  T vbL = uL * bn + vL * byL + wL * bzL, vbLs = SM * bn + vyLs * byLs + vzLs * bzLs;
-  T vbR = uR * bn + vR * byR + wR * bzR, vbRs = SM * bn + vyRs * byRs + vzRs * bzRs;
-  T ELs = ((SL - uL) * EL - ptL * uL + pts * SM + bn * (vbL - vbLs)) * iLM;
-  T ERs = ((SR - uR) * ER - ptR * uR + pts * SM + bn * (vbR - vbRs)) * iRM;
-  // double-star state (glm_physics.jl:288-296)
+  // double-star state (glm_physics.jl:288-296): needs both single-star fans
   T sg = bn >= T(0) ? T(1) : T(-1);
   T isum = rcp(sqL + sqR);
   T vyss = (sqL * vyLs + sqR * vyRs + (byRs - byLs) * sg) * isum;
@@ -256,24 +262,27 @@ __device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fp
   T byss = (sqL * byRs + sqR * byLs + sqL * sqR * (vyRs - vyLs) * sg) * isum;
   T bzss = (sqL * bzRs + sqR * bzLs + sqL * sqR * (vzRs - vzLs) * sg) * isum;
   T vbss = SM * bn + vyss * byss + vzss * bzss;
-  // side selection (glm_physics.jl:297-305)
+  // side selection (glm_physics.jl:297-305); energies and the star energies are needed
+  // on the selected side only
   const bool left = SM >= T(0);
   const T d = left ? dL : dR, u = left ? uL : uR, v = left ? vL : vR, w = left ? wL : wR;
   const T p = left ? pL : pR, by = left ? byL : byR, bz = left ? bzL : bzR;
-  const T E = left ? EL : ER, pt = left ? ptL : ptR;
-  const T S = left ? SL : SR, Ss = left ? SLs : SRs;
+  const T pt = left ? ptL : ptR, b2 = left ? b2L : b2R;
+  const T S = left ? SL : SR, Ss = left ? SLs : SRs, iM = left ? iLM : iRM;
   const T ds = left ? dLs : dRs, vys = left ? vyLs : vyRs, vzs = left ? vzLs : vzRs;
-  const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs, Es = left ? ELs : ERs;
-  const T sq = left ? sqL : sqR, vbs = left ? vbLs : vbRs;
+  const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs;
+  const T sq = left ? sqL : sqR;
+  const T E = p * P.igm1 + T(0.5) * d * (u * u + v * v + w * w) + T(0.5) * b2;
+  const T vb = u * bn + v * by + w * bz;
+  const T vbs = SM * bn + vys * bys + vzs * bzs;
+  const T Es = ((S - u) * E - pt * u + pts * SM + bn * (vb - vbs)) * iM;
   const T Ess = left ? Es - sq * (vbs - vbss) * sg : Es + sq * (vbs - vbss) * sg;
   const bool inner = left ? (SLs < T(0)) : (SRs >= T(0));
   const T a = inner ? Ss : T(0);
   // side physical flux (phys_flux_x of the state with normal field bn)
-  T b2 = left ? b2L : b2R;
   T m = d * u;
-  T vb = u * bn + v * by + w * bz;
   T F0 = m, F1 = m * u + pt - bn2, F2 = m * v - bn * by, F3 = m * w - bn * bz;
-  T F4 = (p * P.igm1 + T(0.5) * d * (u * u + v * v + w * w) + T(0.5) * b2 + pt) * u - bn * vb;
+  T F4 = (E + pt) * u - bn * vb;
   T F6 = u * by - v * bn, F7 = u * bz - w * bn;
   f[0] = F0 + S * (ds - d);
   f[1] = F1 + S * (ds * SM - m);

@@ -363,11 +363,21 @@ __device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
 // global memory (SURR only) and nsmooth the count of network-solved faces.
+// Three waves per SIMD for the float variants whose registers fit 168 VGPRs without
+// spilling (three 41 KB workgroups then share a CU's LDS); the compiler's default
+// (two waves) for the rest.
 template <class T, int RECON, int RS, bool SURR>
-__global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
-                                                         StepParams<T> P, double* __restrict__ red,
-                                                         const unsigned char* __restrict__ simg, float thr,
-                                                         unsigned long long* __restrict__ nsmooth) {
+constexpr bool fused_three_waves() {
+  return sizeof(T) == 4 && !SURR && (RECON == RECON_PLM_MINMOD || (RECON == RECON_PLM && RS != RS_HLLD));
+}
+
+// The single-grid step: one workgroup per work item. simg is the network's image in
+// global memory (SURR only) and nsmooth the count of network-solved faces.
+template <class T, int RECON, int RS, bool SURR>
+__device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
+                                                const StepParams<T>& P, double* __restrict__ red,
+                                                const unsigned char* __restrict__ simg, float thr,
+                                                unsigned long long* __restrict__ nsmooth) {
   __shared__ FusedLds<T> lds;
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
@@ -393,6 +403,33 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fiel
     for (int o = 32; o > 0; o >>= 1) smooth_count += __shfl_xor(smooth_count, o, 64);
     if ((threadIdx.x & 63) == 0 && smooth_count) atomicAdd(nsmooth, (unsigned long long)smooth_count);
   }
+}
+
+template <class T, int RECON, int RS, bool SURR>
+__global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
+                                                         StepParams<T> P, double* __restrict__ red,
+                                                         const unsigned char* __restrict__ simg, float thr,
+                                                         unsigned long long* __restrict__ nsmooth) {
+  step_fused_body<T, RECON, RS, SURR>(U, O, g, P, red, simg, thr, nsmooth);
+}
+
+template <class T, int RECON, int RS, bool SURR>
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused3(
+    Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
+    const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth) {
+  step_fused_body<T, RECON, RS, SURR>(U, O, g, P, red, simg, thr, nsmooth);
+}
+
+template <class T, int RECON, int RS, bool SURR>
+void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
+                          const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
+                          unsigned long long* nsmooth) {
+  if constexpr (fused_three_waves<T, RECON, RS, SURR>())
+    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg, thr,
+                       nsmooth);
+  else
+    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg, thr,
+                       nsmooth);
 }
 
 __global__ __launch_bounds__(256) void k_fold_max_flat(const double* __restrict__ red, int nb, double* __restrict__ out) {
@@ -450,8 +487,7 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
-#define PMF_FUSED(R, S) \
-  hipLaunchKernelGGL((k_step_fused<T, R, S, false>), grd, blk, 0, st, U, O, g, P, red, nullptr, 0.f, nullptr)
+#define PMF_FUSED(R, S) launch_fused_variant<T, R, S, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
   else if (P.rsolve == RS_LLF) PMF_FUSED(R, RS_LLF);     \
@@ -476,8 +512,7 @@ bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& 
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
-#define PMF_FUSED_S(R) \
-  hipLaunchKernelGGL((k_step_fused<float, R, RS_HLLD, true>), grd, blk, 0, st, U, O, g, P, red, simg, thr, nsmooth)
+#define PMF_FUSED_S(R) launch_fused_variant<float, R, RS_HLLD, true>(grd, st, U, O, g, P, red, simg, thr, nsmooth)
   if (P.recon == RECON_PPM) PMF_FUSED_S(RECON_PPM);
   else if (P.recon == RECON_PLM_MINMOD) PMF_FUSED_S(RECON_PLM_MINMOD);
   else PMF_FUSED_S(RECON_PLM);
@@ -565,11 +600,9 @@ __global__ __launch_bounds__(256) void k_ens_params(const EnsCube* __restrict__ 
 }
 
 template <int RECON, int RS>
-__global__ __launch_bounds__(FTHREADS) void k_ens_step(const EnsCube* __restrict__ cubes,
-                                                     const int2* __restrict__ items, EnsQueues q,
-                                                     const StepParams<float>* __restrict__ params,
-                                                     double* __restrict__ red, unsigned* __restrict__ ctr,
-                                                     int parity) {
+__device__ __forceinline__ void ens_step_body(const EnsCube* __restrict__ cubes, const int2* __restrict__ items,
+                                              const EnsQueues& q, const StepParams<float>* __restrict__ params,
+                                              double* __restrict__ red, unsigned* __restrict__ ctr, int parity) {
   __shared__ FusedLds<float> lds;
   __shared__ float sMax[FTHREADS / 64];
   __shared__ int sItem;
@@ -608,6 +641,22 @@ __global__ __launch_bounds__(FTHREADS) void k_ens_step(const EnsCube* __restrict
   }
 }
 
+template <int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) void k_ens_step(const EnsCube* __restrict__ cubes,
+                                                     const int2* __restrict__ items, EnsQueues q,
+                                                     const StepParams<float>* __restrict__ params,
+                                                     double* __restrict__ red, unsigned* __restrict__ ctr,
+                                                     int parity) {
+  ens_step_body<RECON, RS>(cubes, items, q, params, red, ctr, parity);
+}
+
+template <int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_step3(
+    const EnsCube* __restrict__ cubes, const int2* __restrict__ items, EnsQueues q,
+    const StepParams<float>* __restrict__ params, double* __restrict__ red, unsigned* __restrict__ ctr, int parity) {
+  ens_step_body<RECON, RS>(cubes, items, q, params, red, ctr, parity);
+}
+
 FusedGeom ens_geom(int n, int zc) {
   Grid G;
   G.nx = G.ny = G.nz = n;
@@ -629,8 +678,12 @@ void launch_ens_params(const EnsCube* cubes, int ncubes, const double* red, cons
 
 void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q, const StepParams<float>* params,
                      double* red, unsigned* ctr, int nwg, int parity, int recon, int rsolve, hipStream_t st) {
-#define PMF_ENS(R, S) \
-  hipLaunchKernelGGL((k_ens_step<R, S>), dim3(nwg), dim3(FTHREADS), 0, st, cubes, items, q, params, red, ctr, parity)
+#define PMF_ENS(R, S)                                                                                     \
+  if constexpr (fused_three_waves<float, R, S, false>())                                                     \
+    hipLaunchKernelGGL((k_ens_step3<R, S>), dim3(nwg), dim3(FTHREADS), 0, st, cubes, items, q, params, red, ctr, \
+                       parity);                                                                              \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_ens_step<R, S>), dim3(nwg), dim3(FTHREADS), 0, st, cubes, items, q, params, red, ctr, parity)
 #define PMF_ENS_RS(R)                                 \
   if (rsolve == RS_HLL) PMF_ENS(R, RS_HLL);           \
   else if (rsolve == RS_LLF) PMF_ENS(R, RS_LLF);      \
