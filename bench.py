@@ -156,6 +156,67 @@ def ensemble_leg(P):
                       "work_items": nitems}}
 
 
+def timed_run(M, s, steps, integ, cfl=0.2):
+    """`steps` CFL steps (evolve!'s rule with glm_ch_fac 1); wall seconds."""
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt, sm = M.compute_dt(s, cfl)
+        if s.uses_ct:
+            M.step(s, float(dt), integrator=integ)
+        else:
+            M.step(s, float(dt), ch=float(sm), glm_cr=M.GLM_CR, integrator=integ)
+    s.synchronize()
+    return time.perf_counter() - t0
+
+
+def ledger(M, s):
+    tot = M.conserved_totals(s)
+    l2, linf = M.divb_norm(s)
+    return {"totals": tot, "divb_l2": l2, "divb_linf": linf}
+
+
+def c3_leg(M, P, steps):
+    """512^3 plain step (Mach-1.5 turbulence, lognormal density): throughput and the
+    float64 compensated ledger and div B norm before and after."""
+    import numpy as np
+    n = 512
+    U = P.config_state("c3_512", seed=1)
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(U)
+    del U
+    before = ledger(M, s)
+    timed_run(M, s, 2, "cube")  # warm-up (compiles nothing; first-touch of the scratch set)
+    s.timing(True)
+    el = timed_run(M, s, steps, "cube")
+    kms, nl = s.timing_read()
+    s.timing(False)
+    after = ledger(M, s)
+    drift = {k: (after["totals"][k] - before["totals"][k]) / max(abs(before["totals"][k]), 1e-300)
+             for k in ("mass", "energy")}
+    return {"workload": f"plain GLM-HLLD FP32 512^3 (Mach 1.5 solenoidal v, lognormal rho sigma 0.3, |m|<=64, "
+                        f"rms|B| 1), {steps + 2} steps", "value": n ** 3 * steps / el,
+            "kernel_ms": kms / max(nl, 1), "achieved_GBs": 72 * n ** 3 / (kms / max(nl, 1) * 1e-3) / 1e9,
+            "ledger_before": before, "ledger_after": after, "relative_drift": drift,
+            "finite": bool(all(np.isfinite(a).all() for a in s.fields_to_host()))}
+
+
+def ct_leg(M, P, n, steps):
+    """The CT variant (per-cell kernel + GS05 EMF + face curl) on the same inputs as the
+    plain run, after the divergence-free face projection."""
+    import numpy as np
+    U = P.config_state("c2_256", seed=1, n=n)
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod", mhd_solver="ct")
+    s.set_conserved(U)
+    proj = M.ct_faces_from_cells(s)
+    d0 = M.max_divb_face(s)
+    el = timed_run(M, s, steps, "ref")
+    return {"workload": f"CT GLM-free step FP32 {n}^3 on the plain run's inputs, {steps} steps (per-cell :ref kernel)",
+            "value": n ** 3 * steps / el, "projection": {k: float(v) for k, v in proj.items()},
+            "max_divb_face_before": d0, "max_divb_face_after": M.max_divb_face(s),
+            "finite": bool(all(np.isfinite(a).all() for a in s.fields_to_host()))}
+
+
 def traffic_from_profile(n, integ):
     """HBM bytes per fused launch from the committed FETCH_SIZE / WRITE_SIZE summary."""
     path = os.path.join(ROOT, "profiles", "k_step_fused_256cubed_traffic.md")
@@ -182,6 +243,8 @@ def main():
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "pmf-glm_amd", "pmfmhd", "data",
                                                                   "surrogate_weights.npy"))
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
+    ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
                          "size that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -267,6 +330,13 @@ def main():
     if not args.no_ensemble:
         log("bench: ensemble leg")
         ens = ensemble_leg(P)
+    c3 = ct = None
+    if args.c3_steps > 0:
+        log(f"bench: 512^3 leg ({args.c3_steps} steps)")
+        c3 = c3_leg(M, P, args.c3_steps)
+    if args.ct_steps > 0:
+        log(f"bench: CT leg ({args.ct_steps} steps)")
+        ct = ct_leg(M, P, n, args.ct_steps)
 
     kernel = "k_step_fused" if integ == "cube" else "k_step_ref"
     out = {
@@ -293,6 +363,8 @@ def main():
         "error_report": errors,
         "surrogate": sur,
         "ensemble": ens,
+        "c3_512": c3,
+        "ct": ct,
     }
     print(json.dumps(out), flush=True)
 

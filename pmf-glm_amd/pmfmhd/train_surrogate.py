@@ -38,17 +38,20 @@ def dump_runs(net, name, seeds, target, steps, cfl=0.2, thr=THR, log=_log):
     n = P.CONFIGS[name]["n"]
     s = allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod")
     start = net.count
-    keep = None
     per_dump = target / (len(seeds) * steps)
+    # first dump: a keep probability that cannot overflow the target even if every face
+    # passes the gate; later dumps use the gated count it reveals
+    keep0 = float(min(1.0, per_dump / (3 * n ** 3)))
+    keep = None
     for seed in seeds:
         s.set_conserved(P.config_state(name, seed=seed))
         for it in range(steps):
             dt, smax = compute_dt(s, cfl)
             before = net.count
-            k = 1.0 if keep is None else keep
+            k = keep0 if keep is None else keep
             net.dump(s, float(dt), float(smax), thr=thr, keep=k, seed=1000003 * seed + it)
             if keep is None:
-                gated = max(net.count - before, 1)
+                gated = max((net.count - before) / keep0, 1.0)
                 keep = float(min(1.0, per_dump / gated))
             step(s, float(dt), ch=float(smax), integrator="cube")
         log(f"  {name} seed {seed}: {net.count - start} pairs so far (keep {keep:.4g})")

@@ -261,3 +261,55 @@ def test_surrogate_step_matches_float64_oracle(thr):
     # bfloat16 operands: float32-vs-float64 differences can move an input or activation
     # across a bfloat16 rounding boundary (2^-8 relative) in a few faces
     assert q[1] < 1e-5 and q[3] < 5e-3
+
+
+@pytest.mark.gpu
+def test_trained_network_whole_run_error_against_float64():
+    """The committed network (trained by pmfmhd.train_surrogate on seeds 1-16) on a
+    held-out seed: 50 surrogate-assisted steps of the 32^3 benchmark inputs against
+    float64, on the float64 run's dt sequence, next to the plain float32 step and to the
+    method without the network (central flux on the gated faces)."""
+    import os
+
+    import pmfmhd as M
+    from pmfmhd import problems as P
+    from pmfmhd.surrogate import THR, Surrogate, step_surrogate
+    wpath = os.path.join(os.path.dirname(M.__file__), "data", "surrogate_weights.npy")
+    w = np.load(wpath)
+    n, steps = 32, 50
+    U = P.config_state("c1_32", seed=21)
+    s64 = M.allocate_state((n, n, n), 1 / n, dtype=np.float64, recon="plm_minmod")
+    s64.set_conserved(U)
+    seq = []
+    for _ in range(steps):
+        dt, sm = M.compute_dt(s64, 0.2)
+        seq.append((float(dt), float(sm)))
+        M.step(s64, float(dt), ch=float(sm), integrator="cube")
+    ref = s64.conserved()
+
+    def run(weights):
+        s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod")
+        s.set_conserved(U)
+        net = None
+        if weights is not None:
+            net = Surrogate(s, seed=0)
+            net.set_weights(weights)
+        frac = []
+        for k, (dt, ch) in enumerate(seq):
+            if net is None:
+                M.step(s, dt, ch=ch, integrator="cube")
+            else:
+                c = step_surrogate(s, net, dt, ch, thr=THR, count=(k % 10 == 0))
+                if c is not None:
+                    frac.append(c / (3 * n ** 3))
+        got = s.conserved().astype(np.float64)
+        return np.linalg.norm((got[:8] - ref[:8]).ravel()) / np.linalg.norm(ref[:8].ravel()), frac
+
+    e_plain, _ = run(None)
+    e_sur, frac = run(w)
+    e_central, _ = run(np.zeros_like(w))
+    print(f"whole-run rel-L2 vs float64 after {steps} steps: plain {e_plain:.3e}, surrogate {e_sur:.3e}, "
+          f"central-only {e_central:.3e}; network face fraction {np.round(frac, 3)}")
+    assert min(frac) > 0.02
+    assert e_plain < 1e-5
+    assert e_sur < 0.25 * e_central
