@@ -93,27 +93,27 @@ __device__ __forceinline__ bool surr_prepare(T* L, T* R, const StepParams<T>& P,
          (P.llf_pmin > T(0) && dmin(L[4], R[4]) < P.llf_pmin);
 }
 
-// tanh-form GELU and its derivative (used identically in training and inference).
+// tanh-form GELU, 0.5 z (1 + tanh(u)) with u = k0 (z + k1 z^3), evaluated as
+// z * sigmoid(2u) = z / (1 + exp(-2u)) (one exp2, one reciprocal), and its derivative
+// sigma + 2 z sigma (1 - sigma) k0 (1 + 3 k1 z^2); used identically in training and
+// inference.
 __device__ __forceinline__ float gelu(float z) {
-  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  float u = k0 * (z + k1 * z * z * z);
-  float e = __builtin_amdgcn_exp2f(2.0f * 1.4426950408889634f * u);  // exp(2u)
-  float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);            // tanh(u)
-  return 0.5f * z * (1.0f + t);
+  const float c1 = -2.0f * 1.4426950408889634f * 0.7978845608028654f, c3 = c1 * 0.044715f;
+  const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(c3, z * z, c1));  // exp(-2u)
+  return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ __forceinline__ float gelu_grad(float z) {
   const float k0 = 0.7978845608028654f, k1 = 0.044715f;
-  float u = k0 * (z + k1 * z * z * z);
-  float e = __builtin_amdgcn_exp2f(2.0f * 1.4426950408889634f * u);
-  float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * z * z);
+  const float c1 = -2.0f * 1.4426950408889634f * k0, c3 = c1 * k1;
+  const float z2 = z * z;
+  const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(c3, z2, c1));
+  const float sg = __builtin_amdgcn_rcpf(1.0f + e);
+  return sg + 2.0f * z * sg * (1.0f - sg) * k0 * (1.0f + 3.0f * k1 * z2);
 }
 
-// float -> bfloat16 bits, round to nearest even (finite inputs).
+// float -> bfloat16, round to nearest even (the hardware v_cvt_pk_bf16_f32).
 __device__ __forceinline__ unsigned short bf16_bits(float f) {
-  unsigned u = __builtin_bit_cast(unsigned, f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ __forceinline__ float bf16_round(float f) {
   unsigned u = (unsigned)bf16_bits(f) << 16;
@@ -136,69 +136,82 @@ constexpr int SIMG_NW = SIMG_W4 + 32 * SIMG_S2;  // bf16 entries: 12160
 constexpr int SIMG_NB = 64 * 3 + 32;             // float biases (output padded)
 constexpr int SIMG_BYTES = SIMG_NW * 2 + SIMG_NB * 4;
 
-typedef short sbf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef float sf32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ sbf16x4 lds_frag(const unsigned short* base) {
-  return *reinterpret_cast<const sbf16x4*>(base);
+// Eight bfloat16 weights of one row of the LDS image: four at column c0, four at c1.
+__device__ __forceinline__ bf16x8 lds_frag8(const unsigned short* row, int c0, int c1) {
+  const u16x4 a = *reinterpret_cast<const u16x4*>(row + c0);
+  const u16x4 b = *reinterpret_cast<const u16x4*>(row + c1);
+  s16x8 v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[q] = (short)a[q];
+    v[4 + q] = (short)b[q];
+  }
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// Accumulator initialised with the layer's biases: register r of lane half h holds unit
+// 32 mb + 8 (r / 4) + 4 h + r % 4 (the 32x32 MFMA output layout).
+__device__ __forceinline__ sf32x16 bias_acc(const float* __restrict__ b, int mb, int lh) {
+  sf32x16 d;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) d[r] = b[32 * mb + 8 * (r / 4) + 4 * lh + r % 4];
+  return d;
 }
 
 // y[9] = MLP(x[16]) for the 64 faces held by the 64 lanes of the calling wave (one face
 // per lane). Must be called by all lanes of the wave. Units are the MFMA rows and faces
-// the columns (32 per pass), so a layer's 32x32 accumulator registers 4s..4s+3 are
-// exactly the next layer's B fragment for k-step s: activations stay in registers.
+// the columns (32 per pass); v_mfma_f32_32x32x16_bf16 contracts 16 units per step. The
+// hidden layers' contraction order is permuted so that accumulator registers 8s..8s+7
+// of a lane are exactly its B fragment for step s of the next layer (lane half h then
+// holds units 16s + 4h + {0..3} and 16s + 8 + 4h + {0..3}); the weight fragments read
+// the same units. Activations never leave registers.
 __device__ __forceinline__ void mlp_wave(const float* x, float* y, const unsigned short* __restrict__ sw,
                                          const float* __restrict__ sb) {
   const int lane = threadIdx.x & 63, li = lane & 31, lh = lane >> 5;
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass) {
     const int src = pass * 32 + li;
-    sbf16x4 bin[2];
+    // layer 1: inputs 8h..8h+7 of face (pass, li) in lane half h
+    bf16x8 bin;
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float lo = __shfl(x[8 * ks + q], src, 64), hi = __shfl(x[8 * ks + 4 + q], src, 64);
-        bin[ks][q] = (short)bf16_bits(lh ? hi : lo);
-      }
+    for (int q = 0; q < 8; ++q) {
+      const float lo = __shfl(x[q], src, 64), hi = __shfl(x[8 + q], src, 64);
+      bin[q] = (__bf16)(lh ? hi : lo);
+    }
     sf32x16 d[2];
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) d[mb][r] = 0.f;
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks)
-        d[mb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-            lds_frag(sw + SIMG_W1 + (mb * 32 + li) * SIMG_S1 + 8 * ks + 4 * lh), bin[ks], d[mb], 0, 0, 0);
-    }
+    for (int mb = 0; mb < 2; ++mb)
+      d[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+          lds_frag8(sw + SIMG_W1 + (mb * 32 + li) * SIMG_S1, 8 * lh, 8 * lh + 4), bin, bias_acc(sb, mb, lh), 0, 0, 0);
 #pragma unroll 1
     for (int layer = 1; layer < 4; ++layer) {
-      sbf16x4 b[8];
-      const float* bias = sb + 64 * (layer - 1);
+      bf16x8 b[4];
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks)
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = 4 * (ks & 3) + q;
-          const int u = (ks >> 2) * 32 + 8 * (ks & 3) + 4 * lh + q;
-          b[ks][q] = (short)bf16_bits(gelu(d[ks >> 2][r] + bias[u]));
-        }
+        for (int q = 0; q < 8; ++q) b[st][q] = (__bf16)gelu(d[st >> 1][8 * (st & 1) + q]);
       const int off = layer == 1 ? SIMG_W2 : layer == 2 ? SIMG_W3 : SIMG_W4;
       const int nmb = layer == 3 ? 1 : 2;
+      const float* bias = sb + 64 * layer;
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
         if (mb >= nmb) break;
+        sf32x16 acc = bias_acc(bias, mb, lh);
+        const unsigned short* row = sw + off + (mb * 32 + li) * SIMG_S2;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) d[mb][r] = 0.f;
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks)
-          d[mb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-              lds_frag(sw + off + (mb * 32 + li) * SIMG_S2 + 8 * ks + 4 * lh), b[ks], d[mb], 0, 0, 0);
+        for (int st = 0; st < 4; ++st)
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag8(row, 16 * st + 4 * lh, 16 * st + 8 + 4 * lh), b[st],
+                                                         acc, 0, 0, 0);
+        d[mb] = acc;
       }
     }
     // outputs o of face column j: o = 0..3 in lane j (registers 0..3), o = 4..7 in lane
     // j+32 (registers 0..3), o = 8 in lane j (register 4)
-    const float* b4 = sb + 192;
     float o[9];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -208,7 +221,7 @@ __device__ __forceinline__ void mlp_wave(const float* x, float* y, const unsigne
     o[8] = __shfl(d[0][4], li, 64);
     if (lh == pass) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) y[k] = o[k] + b4[k];
+      for (int k = 0; k < 9; ++k) y[k] = o[k];
     }
   }
 }
