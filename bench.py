@@ -243,6 +243,7 @@ def main():
     ap.add_argument("--surrogate-weights", default=os.path.join(ROOT, "pmf-glm_amd", "pmfmhd", "data",
                                                                   "surrogate_weights.npy"))
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in z-chunk depth")
     ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
     ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -275,6 +276,13 @@ def main():
         except M.MHDError as e:
             log(f"bench: fused kernel unavailable ({e}); using the per-cell kernel")
             integ = "ref"
+        s.set_conserved(U)
+
+    tuned = None
+    if integ == "cube" and not args.no_autotune:
+        zc, ms = M.autotune(s, reps=5)
+        tuned = {"zc": zc, "ms": ms}
+        log(f"bench: autotuned z chunk {zc} planes ({ms:.3f} ms/step)")
         s.set_conserved(U)
 
     def one_step():
@@ -355,7 +363,7 @@ def main():
                 "rho = 1 + 0.01 N(0,1), p = 1",
         "config": {"workload": f"plain GLM-HLLD FP32 {n}^3, PLM minmod, CFL 0.2, gamma 5/3, periodic",
                    "n": n, "recon": "plm_minmod", "riemann": "hlld", "cfl": 0.2, "integrator": integ,
-                   "parallelism": "single GPU", "finite": bool(finite)},
+                   "parallelism": "single GPU", "finite": bool(finite), "autotuned_z_chunk": tuned},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n, integ),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_launch": BYTES_PER_CELL * cells},

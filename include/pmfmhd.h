@@ -123,6 +123,12 @@ int pmf_ct_faces_from_potential(pmf_state* s, const void* ax, const void* ay, co
 int pmf_riemann_flux_batch(pmf_state* s, const void* L, const void* R, int32_t dir, double ch,
                            int64_t n, void* F);
 
+/* Tile depth of the fused (:cube) kernel: planes per z chunk, at least 4 (0 = the
+ * built-in choice, the default). Results do not depend on it. */
+int pmf_set_chunk(pmf_state* s, int32_t zc);
+/* Times the fused kernel for z chunks of 4..128 planes (reps launches each, writing
+ * only the scratch fields) and keeps the fastest; returns it and its time per step. */
+int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
 /* Kernel timing: while enabled, HIP events bracket every step-kernel launch on the
  * state's stream; pmf_timing_read waits for the stream and returns the summed kernel
  * time (ms) and the number of timed launches since the last enable. */

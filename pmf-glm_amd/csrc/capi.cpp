@@ -60,7 +60,8 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
                   "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
                   ")");
     mark(s);
-    bool done = launch_step_fused<T>(in, out, s->G, make_params<T>(s, dt, ch, decay), s->red, s->dval, s->st);
+    bool done =
+        launch_step_fused<T>(in, out, s->G, make_params<T>(s, dt, ch, decay), s->red, s->dval, s->st, s->zc);
     mark(s);
     if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
   } else {
@@ -89,7 +90,6 @@ void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
   s->smax_valid = integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM;
 }
 
-// decay = exp(-T(glm_cr)*T(ch)*T(dt)/dx) in the state's precision (driver.jl:29).
 }  // namespace
 
 extern "C" {
@@ -148,7 +148,7 @@ void create_state(const pmf_params* pp, pmf_state** out, bool with_fields) {
         hip_check(hipMemsetAsync(s->ct, 0, 21 * s->nc * s->esz, s->st), "hipMemset");
       }
     }
-    size_t nred = std::max<size_t>(size_t(kRedBlocks) * kRedSlots, size_t(fused_blocks(s->G)));
+    size_t nred = std::max<size_t>(size_t(kRedBlocks) * kRedSlots, size_t(fused_blocks(s->G, kMinChunk)));
     hip_check(hipMalloc(&s->red, sizeof(double) * nred), "hipMalloc(red)");
     hip_check(hipMalloc(&s->dval, sizeof(double) * 64), "hipMalloc(dval)");
     hip_check(hipHostMalloc((void**)&s->hval, sizeof(double) * 64, hipHostMallocDefault), "hipHostMalloc");
@@ -336,6 +336,69 @@ int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, doubl
     }
     if (t_out) *t_out = t;
     if (n_out) *n_out = n;
+  });
+}
+
+int pmf_set_chunk(pmf_state* s, int32_t zc) {
+  return guard([&] {
+    check_state(s);
+    if (zc != 0 && zc < kMinChunk) throw Error("z chunks must be at least " + std::to_string(kMinChunk) +
+                                               " planes (or 0 for the built-in choice)");
+    s->zc = zc;
+  });
+}
+
+int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out) {
+  return guard([&] {
+    check_fields(s);
+    if (reps < 1) throw Error("reps must be positive");
+    if (s->p.solver != PMF_SOLVER_GLM || !fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
+      throw Error("autotune: the fused kernel does not run on this state");
+    double dt, smax;
+    if (pmf_compute_dt(s, 0.2, &dt, &smax)) throw Error(g_err);
+    const double ch = smax > 0 ? smax : 1.0;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    const int keep = s->zc;
+    int best_zc = 0;
+    double best = 1e300;
+    try {
+      for (int zc : {4, 8, 12, 16, 24, 32, 48, 64, 96, 128}) {
+        if (zc > s->G.nz && zc != 4) break;
+        s->zc = zc;
+        auto launch = [&]() {
+          if (s->p.precision == PMF_F64)
+            step_fields<double>(s, cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc), nullptr, dt, ch,
+                                1.0, PMF_INTEGRATOR_CUBE);
+          else
+            step_fields<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), nullptr, dt, ch, 1.0,
+                               PMF_INTEGRATOR_CUBE);
+        };
+        launch();  // warm-up
+        hip_check(hipEventRecord(e0, s->st), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) launch();
+        hip_check(hipEventRecord(e1, s->st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ms / reps < best) {
+          best = ms / reps;
+          best_zc = zc;
+        }
+      }
+    } catch (...) {
+      s->zc = keep;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->zc = best_zc;
+    s->smax_valid = false;  // dval now holds the speed of the scratch fields
+    if (zc_out) *zc_out = best_zc;
+    if (ms_out) *ms_out = best;
   });
 }
 

@@ -18,17 +18,20 @@ void launch_step_ref(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, cons
 // Fused tiled step (periodic GLM). Writes per-block CFL maxima of the new state into
 // red (one double per block) and the final max into dsmax. Returns false when the grid
 // shape is not supported by the fused kernel.
+// zc: planes per z chunk (0 = the built-in choice).
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st);
+                       double* dsmax, hipStream_t st, int zc = 0);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
 // The surrogate-assisted fused step (float32, HLLD): simg is the network's LDS image,
 // nsmooth accumulates the number of faces solved by the network.
 bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
                             double* red, double* dsmax, const unsigned char* simg, float thr,
                             unsigned long long* nsmooth, hipStream_t st);
-// Number of workgroups (one partial maximum each) the fused kernel launches for G.
-int64_t fused_blocks(const Grid& G);
+// Number of workgroups (one partial maximum each) the fused kernel launches for G with
+// z chunks of zc planes (0 = the built-in choice).
+int64_t fused_blocks(const Grid& G, int zc = 0);
+constexpr int kMinChunk = 4;  // smallest z chunk the library accepts
 
 // Geometry of the fused kernel's work decomposition: 16x12 column tiles (x fastest,
 // then y; the last tile row may be ragged) marched through z chunks of zc planes.

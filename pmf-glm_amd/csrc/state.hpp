@@ -52,6 +52,7 @@ struct pmf_state {
   hipStream_t st = nullptr;
   int device = 0;
   bool smax_valid = false;  // dval[0] holds the CFL speed of the current U
+  int zc = 0;               // planes per z chunk of the fused kernel (0 = built-in choice)
   bool timing = false;      // bracket step kernels with events
   std::vector<hipEvent_t> ev;  // pairs (start, stop) in launch order
   size_t ev_used = 0;

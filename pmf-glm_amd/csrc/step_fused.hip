@@ -474,17 +474,19 @@ static int pick_zc(const Grid& G) {
   return zc < G.nz ? zc : G.nz;
 }
 
-int64_t fused_blocks(const Grid& G) {
+static int chunk_of(const Grid& G, int zc) { return zc > 0 ? (zc < G.nz ? zc : G.nz) : pick_zc(G); }
+
+int64_t fused_blocks(const Grid& G, int zc) {
   if (G.nx % TX) return 0;
-  FusedGeom g = fused_geom(G, pick_zc(G));
+  FusedGeom g = fused_geom(G, chunk_of(G, zc));
   return (int64_t)g.ntx * g.nty * g.nchunks;
 }
 
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st) {
+                       double* dsmax, hipStream_t st, int zc) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
-  FusedGeom g = fused_geom(G, pick_zc(G));
+  FusedGeom g = fused_geom(G, chunk_of(G, zc));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
 #define PMF_FUSED(R, S) launch_fused_variant<T, R, S, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
@@ -700,8 +702,8 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 }
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
-                                       double*, double*, hipStream_t);
+                                       double*, double*, hipStream_t, int);
 template bool launch_step_fused<double>(Fields<const double>, Fields<double>, const Grid&, const StepParams<double>&,
-                                        double*, double*, hipStream_t);
+                                        double*, double*, hipStream_t, int);
 
 }  // namespace pmf

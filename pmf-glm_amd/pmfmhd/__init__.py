@@ -228,6 +228,19 @@ def max_wavespeed(s: MHDState):
     return s.dtype.type(v.value)
 
 
+def set_chunk(s: MHDState, zc: int):
+    """Planes per z chunk of the fused kernel (0 = built-in choice); results do not depend on it."""
+    check(lib().pmf_set_chunk(s.handle, int(zc)))
+
+
+def autotune(s: MHDState, reps: int = 5):
+    """Time the fused kernel over z-chunk depths on this state (scratch is overwritten) and
+    keep the fastest; returns (zc, ms per step)."""
+    zc, ms = ctypes.c_int32(), ctypes.c_double()
+    check(lib().pmf_autotune(s.handle, int(reps), ctypes.byref(zc), ctypes.byref(ms)))
+    return zc.value, ms.value
+
+
 def compute_dt(s: MHDState, cfl=0.4):
     """timestep.jl:43-46 -> (dt, smax) in the state's precision."""
     dt, sm = ctypes.c_double(), ctypes.c_double()

@@ -70,6 +70,8 @@ def lib():
             "pmf_surrogate_train": [P, i32, i32, d, ctypes.c_uint64, i32, P],
             "pmf_surrogate_face_flux": [P, i64, i64, i32, d, P],
             "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
+            "pmf_set_chunk": [P, i32],
+            "pmf_autotune": [P, i32, ctypes.POINTER(ctypes.c_int32), pd],
             "pmf_ensemble_create": [ctypes.POINTER(pmf_params), i32, P, ctypes.POINTER(ctypes.c_void_p)],
             "pmf_ensemble_destroy": [P],
             "pmf_ensemble_upload": [P, i32, i32, P, i64],

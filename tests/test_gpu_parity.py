@@ -367,3 +367,36 @@ def test_step_into_caller_owned_arrays_is_bitwise_the_state_step(integrator):
     finally:
         ins.free()
         outs.free()
+
+
+@pytest.mark.gpu
+def test_fused_chunk_depth_and_autotune_leave_results_unchanged():
+    """The z-chunk depth only regroups work: every depth gives the same bits, and the
+    autotuner picks one of its candidates without touching the state."""
+    import pmfmhd as M
+    from pmfmhd import problems as P
+    U = P.config_state("c1_32", seed=3, n=48)
+    n = 48
+    outs = []
+    for zc in (0, 4, 7, 48, 200):
+        s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod")
+        s.set_conserved(U)
+        M.set_chunk(s, zc)
+        for _ in range(3):
+            dt, sm = M.compute_dt(s, 0.2)
+            M.step(s, float(dt), ch=float(sm), integrator="cube")
+        outs.append(s.conserved())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(U)
+    before = s.conserved()
+    zc, ms = M.autotune(s, reps=2)
+    assert zc in (4, 8, 12, 16, 24, 32, 48) and ms > 0
+    assert np.array_equal(s.conserved(), before)
+    for _ in range(3):
+        dt, sm = M.compute_dt(s, 0.2)
+        M.step(s, float(dt), ch=float(sm), integrator="cube")
+    assert np.array_equal(s.conserved(), outs[0])
+    with pytest.raises(M.MHDError, match="at least 4"):
+        M.set_chunk(s, 2)
