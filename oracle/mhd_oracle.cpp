@@ -347,21 +347,191 @@ inline int64_t pidx(const Grid& G, int64_t i, int64_t j, int64_t k) {
          bcidx(i, G.nx, BC_PERIODIC, f);
 }
 
+// ---------------------------------------------------------------------------
+// Source terms of the drag / radiation-coupled steps (step_ref_drag!,
+// step_ref_radiation!, step_cube_drag!, step_cube_radiation!).
+template <class T> struct Src {
+  bool drag = false, rad = false, track_rho = false, track_b = false, track_e = false;
+  int check = 0;  // 0 none, 1 admissibility flag, 2 masked fallback (slot 4 > 0), 3 global fallback
+  T q = 0;        // drag impulse Gamma dt
+  T vx0 = 0, vy0 = 0, vz0 = 0;
+  T* corr = nullptr;  // 9 slots of nc: psi, dv_x, dv_y, dv_z, rho perturbation (or mask), B_x, B_y, B_z, E residuals
+};
+
+// glm_physics.jl:63-65
+template <class T> inline T drag_phi1(T q) {
+  return std::fabs(q) < T(1e-4) ? T(1) - q / T(2) + q * q / T(6) : -std::expm1(-q) / q;
+}
+
+// glm_physics.jl:87-103: coefficients of Gamma * int_0^dt |w(t)|^2 dt
+template <class T> inline void drag_work(T q, T& A, T& C, T& D) {
+  if (q < T(0.05)) {
+    T q2 = q * q, q3 = q2 * q, q4 = q3 * q;
+    A = q * (T(1) - q + T(2.0 / 3.0) * q2 - T(1.0 / 3.0) * q3 + T(2.0 / 15.0) * q4);
+    C = q * (T(1) - q + T(7.0 / 12.0) * q2 - T(1.0 / 4.0) * q3 + T(31.0 / 360.0) * q4);
+    D = q * (T(1.0 / 3.0) - q / T(4) + T(7.0 / 60.0) * q2 - q3 / T(24) + T(31.0 / 2520.0) * q4);
+    return;
+  }
+  T em1 = -std::expm1(-q), em2 = -std::expm1(-T(2) * q);
+  A = T(0.5) * em2;
+  C = em1 * em1 / q;
+  D = (q - T(2) * em1 + T(0.5) * em2) / (q * q);
+}
+
+// glm_physics.jl:71-82 (drag_predict_face with the impulse, the target velocity v0)
+template <class T>
+inline Q<T> drag_predict_face(const Q<T>& q0, const Q<T>& qh, T impulse, T vx0, T vy0, T vz0) {
+  T decay = std::exp(-impulse), phi1 = drag_phi1(impulse);
+  Q<T> r = qh;
+  r.v[1] = vx0 + decay * (q0.v[1] - vx0) + phi1 * (qh.v[1] - q0.v[1]);
+  r.v[2] = vy0 + decay * (q0.v[2] - vy0) + phi1 * (qh.v[2] - q0.v[2]);
+  r.v[3] = vz0 + decay * (q0.v[3] - vz0) + phi1 * (qh.v[3] - q0.v[3]);
+  return r;
+}
+
+// integrator_ref.jl:33-39 (_facep_sources): drag-propagated face velocity (half-step
+// impulse), plus the radiation velocity correction of the predicted cell.
+template <class T>
+inline Q<T> facep_sources(const Q<T>& m0, const Q<T>& mh, const Q<T>& d, const Src<T>& S, int64_t idx, int64_t nc) {
+  Q<T> q = S.drag ? drag_predict_face(padd(m0, d), padd(mh, d), T(0.5) * S.q, S.vx0, S.vy0, S.vz0) : padd(mh, d);
+  if (S.rad) {
+    q.v[1] = q.v[1] + S.corr[nc + idx];
+    q.v[2] = q.v[2] + S.corr[2 * nc + idx];
+    q.v[3] = q.v[3] + S.corr[3 * nc + idx];
+  }
+  return q;
+}
+
+// glm_physics.jl:350-372: x + dx with a carried float residual (Kahan), optionally floored.
+template <class T> inline T comp_update(T* res, T x, T dx, bool tracked) {
+  if (!tracked) return x + dx;
+  T carry = *res;
+  T increment = dx + carry;
+  T updated = x + increment;
+  *res = increment - (updated - x);
+  return updated;
+}
+template <class T> inline T comp_floor_update(T* res, T x, T dx, T floor, bool tracked) {
+  if (!tracked) return std::max(x + dx, floor);
+  T updated = comp_update(res, x, dx, true);
+  if (updated < floor) {
+    *res = T(0);
+    return floor;
+  }
+  return updated;
+}
+
+// glm_physics.jl:143-170 and 383-390: the exact constant-force propagator applied to the
+// conservative trial state; total energy loses the integrated drag work.
+template <class T>
+inline Q<T> drag_correct(const Q<T>& u0, const Q<T>& tr, T q, T vx0, T vy0, T vz0, T smallr, T smallE, T* eres,
+                         bool tracked) {
+  T decay = std::exp(-q), phi1 = drag_phi1(q), A, C, D;
+  drag_work(q, A, C, D);
+  T rn = std::max(tr.v[0], smallr), ro = std::max(u0.v[0], smallr);
+  T vox = u0.v[1] / ro, voy = u0.v[2] / ro, voz = u0.v[3] / ro;
+  T vtx = tr.v[1] / rn, vty = tr.v[2] / rn, vtz = tr.v[3] / rn;
+  T dvx = vtx - vox, dvy = vty - voy, dvz = vtz - voz;
+  T wx = vox - vx0, wy = voy - vy0, wz = voz - vz0;
+  T vnx = vx0 + decay * wx + phi1 * dvx;
+  T vny = vy0 + decay * wy + phi1 * dvy;
+  T vnz = vz0 + decay * wz + phi1 * dvz;
+  T w2 = wx * wx + wy * wy + wz * wz;
+  T wdv = wx * dvx + wy * dvy + wz * dvz;
+  T dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+  T iwx = (T(1) - decay) * wx + (T(1) - phi1) * dvx;
+  T iwy = (T(1) - decay) * wy + (T(1) - phi1) * dvy;
+  T iwz = (T(1) - decay) * wz + (T(1) - phi1) * dvz;
+  T qwork = rn * (A * w2 + C * wdv + D * dv2 + vx0 * iwx + vy0 * iwy + vz0 * iwz);
+  T nmx = rn * vnx, nmy = rn * vny, nmz = rn * vnz;
+  T ek = T(0.5) * (nmx * nmx + nmy * nmy + nmz * nmz) / rn;
+  T eb = T(0.5) * (tr.v[5] * tr.v[5] + tr.v[6] * tr.v[6] + tr.v[7] * tr.v[7]);
+  T emin = ek + eb + smallE;
+  Q<T> r = tr;
+  r.v[0] = rn;
+  r.v[1] = nmx;
+  r.v[2] = nmy;
+  r.v[3] = nmz;
+  r.v[4] = comp_floor_update(eres, tr.v[4], -qwork, emin, tracked);
+  return r;
+}
+
+// integrator_cube.jl:100-116
+template <class T> inline bool inadmissible(const Q<T>& rh, T smallr, T pfl, T g, T llf_dmin) {
+  T rho = rh.v[0];
+  T gate = std::max(smallr, llf_dmin);
+  T ir = T(1) / std::max(rho, smallr);
+  T ek = T(0.5) * (rh.v[1] * rh.v[1] + rh.v[2] * rh.v[2] + rh.v[3] * rh.v[3]) * ir;
+  T eb = T(0.5) * (rh.v[5] * rh.v[5] + rh.v[6] * rh.v[6] + rh.v[7] * rh.v[7]);
+  T eint = rh.v[4] - ek - eb;
+  T emin = pfl / (g - T(1));
+  bool finite = std::isfinite(rho) && std::isfinite(eint);
+  for (int v = 1; v < 9; ++v) finite = finite && std::isfinite(rh.v[v]);
+  return !finite || rho < gate || eint < emin;
+}
+
 // integrator_ref.jl:69-160 for one cell (GLM, or CT when ctp != nullptr; no drag /
 // radiation / tracking: the plain step of step_ref!, integrator_ref.jl:210-220).
+//
+// With sources (S != nullptr): the drag / radiation / tracking / admissibility terms of
+// integrator_ref.jl:28-48,150-160 and, for `check`, of the cube kernel
+// (integrator_cube.jl:68-116,330-340,423-441): a masked or global fallback zeroes the
+// slopes of flagged cells and solves every face touching one with LLF.
+template <class T>
+inline bool src_masked(const Src<T>* S, const Grid& G, int64_t i, int64_t j, int64_t k) {
+  if (!S || S->check < 2) return false;
+  if (S->check == 3) return true;
+  const int64_t nc = G.nx * G.ny * G.nz;
+  return S->corr[4 * nc + pidx(G, i, j, k)] > T(0);
+}
+
+template <class T>
+inline Edges<T> mh_edges_src(const T* const* u, const Grid& G, int64_t i, int64_t j, int64_t k, T dtdx,
+                             const Phys<T>& P, const Src<T>* S) {
+  if (!src_masked(S, G, i, j, k)) return mh_edges(u, G, i, j, k, dtdx, P);
+  Edges<T> e;
+  e.m0 = loadP(u, G, i, j, k, P);
+  for (int d = 0; d < 3; ++d)
+    for (int v = 0; v < 9; ++v) e.dL[d].v[v] = e.dR[d].v[v] = T(0);
+  Q<T> uh = hancock_edges(e.m0, e.dL, e.dR, dtdx, P.gamma);
+  e.mh = cons2prim(uh, P.gamma, P.smallr, P.pfl);
+  return e;
+}
+
 template <class T>
 void step_cell(const T* const* u, T* const* o, const Grid& G, int64_t c, T dtdx, T ch, T decay,
-               const Phys<T>& P, T* ctp) {
+               const Phys<T>& P, T* ctp, const Src<T>* S = nullptr) {
   const int64_t nc = G.nx * G.ny * G.nz;
   int64_t i = c % G.nx, j = (c / G.nx) % G.ny, k = c / (G.nx * G.ny);
-  Edges<T> ec = mh_edges(u, G, i, j, k, dtdx, P);
+  Edges<T> ec = mh_edges_src(u, G, i, j, k, dtdx, P, S);
   Q<T> F[3][2];  // [dir][lo, hi]
   const int64_t di[3] = {1, 0, 0}, dj[3] = {0, 1, 0}, dk[3] = {0, 0, 1};
+  const bool mc = src_masked(S, G, i, j, k);
+  Phys<T> PL = P;
+  PL.rsolve = RS_LLF;
   for (int d = 0; d < 3; ++d) {
-    Edges<T> em = mh_edges(u, G, i - di[d], j - dj[d], k - dk[d], dtdx, P);
-    Edges<T> ep = mh_edges(u, G, i + di[d], j + dj[d], k + dk[d], dtdx, P);
-    Q<T> Ll = padd(em.mh, em.dR[d]), Rl = padd(ec.mh, ec.dL[d]);
-    Q<T> Lh = padd(ec.mh, ec.dR[d]), Rh = padd(ep.mh, ep.dL[d]);
+    Edges<T> em = mh_edges_src(u, G, i - di[d], j - dj[d], k - dk[d], dtdx, P, S);
+    Edges<T> ep = mh_edges_src(u, G, i + di[d], j + dj[d], k + dk[d], dtdx, P, S);
+    Q<T> Ll, Rl, Lh, Rh;
+    if (S) {
+      const int64_t im = pidx(G, i - di[d], j - dj[d], k - dk[d]), ip = pidx(G, i + di[d], j + dj[d], k + dk[d]);
+      Ll = facep_sources(em.m0, em.mh, em.dR[d], *S, im, nc);
+      Rl = facep_sources(ec.m0, ec.mh, ec.dL[d], *S, c, nc);
+      Lh = facep_sources(ec.m0, ec.mh, ec.dR[d], *S, c, nc);
+      Rh = facep_sources(ep.m0, ep.mh, ep.dL[d], *S, ip, nc);
+    } else {
+      Ll = padd(em.mh, em.dR[d]);
+      Rl = padd(ec.mh, ec.dL[d]);
+      Lh = padd(ec.mh, ec.dR[d]);
+      Rh = padd(ep.mh, ep.dL[d]);
+    }
+    if (S && S->check == 2) {
+      const bool ml = src_masked(S, G, i - di[d], j - dj[d], k - dk[d]);
+      const bool mp = src_masked(S, G, i + di[d], j + dj[d], k + dk[d]);
+      F[d][0] = riemann(Ll, Rl, d, ch, (ml || mc) ? PL : P);
+      F[d][1] = riemann(Lh, Rh, d, ch, (mc || mp) ? PL : P);
+      continue;
+    }
     if (ctp) {
       // integrator_ref.jl:83-87,100-104,117-121: the staggered face value is the
       // normal field of both states.
@@ -382,28 +552,59 @@ void step_cell(const T* const* u, T* const* o, const Grid& G, int64_t c, T dtdx,
     ctp[6 * nc + c] = F[0][0].v[6]; ctp[7 * nc + c] = F[0][0].v[7];
     ctp[8 * nc + c] = F[1][0].v[5]; ctp[9 * nc + c] = F[1][0].v[7];
     ctp[10 * nc + c] = F[2][0].v[5]; ctp[11 * nc + c] = F[2][0].v[6];
-    const Q<T>& qc = ec.mh;
+    Q<T> zero{};
+    const Q<T> qc = S ? facep_sources(ec.m0, ec.mh, zero, *S, c, nc) : ec.mh;
     ctp[12 * nc + c] = qc.v[3] * U0[6] - qc.v[2] * U0[7];
     ctp[13 * nc + c] = qc.v[1] * U0[7] - qc.v[3] * U0[5];
     ctp[14 * nc + c] = qc.v[2] * U0[5] - qc.v[1] * U0[6];
   }
   // integrator_ref.jl:140-158: U0 + dtdx*((Fxl-Fxh)+(Fyl-Fyh)+(Fzl-Fzh)), psi*decay
-  for (int v = 0; v < 9; ++v) {
-    T du = dtdx * ((F[0][0].v[v] - F[0][1].v[v]) + (F[1][0].v[v] - F[1][1].v[v]) + (F[2][0].v[v] - F[2][1].v[v]));
-    T r = U0[v] + du;
-    if (v == 8) r = ctp ? T(0) : r * decay;
-    o[v][c] = r;
+  T du[9];
+  for (int v = 0; v < 9; ++v)
+    du[v] = dtdx * ((F[0][0].v[v] - F[0][1].v[v]) + (F[1][0].v[v] - F[1][1].v[v]) + (F[2][0].v[v] - F[2][1].v[v]));
+  if (!S) {
+    for (int v = 0; v < 9; ++v) {
+      T r = U0[v] + du[v];
+      if (v == 8) r = ctp ? T(0) : r * decay;
+      o[v][c] = r;
+    }
+    return;
   }
+  // integrator_ref.jl:150-160 / integrator_cube.jl:423-441 with tracking and drag
+  T* cr = S->corr;
+  Q<T> rh;
+  if (S->track_rho) cr[4 * nc + c] = cr[4 * nc + c] + du[0];
+  rh.v[0] = U0[0] + du[0];
+  rh.v[1] = U0[1] + du[1];
+  rh.v[2] = U0[2] + du[2];
+  rh.v[3] = U0[3] + du[3];
+  rh.v[4] = comp_update(cr ? &cr[8 * nc + c] : nullptr, U0[4], du[4], S->track_e);
+  const bool tb = S->track_b && !ctp;
+  rh.v[5] = comp_update(cr ? &cr[5 * nc + c] : nullptr, U0[5], du[5], tb);
+  rh.v[6] = comp_update(cr ? &cr[6 * nc + c] : nullptr, U0[6], du[6], tb);
+  rh.v[7] = comp_update(cr ? &cr[7 * nc + c] : nullptr, U0[7], du[7], tb);
+  rh.v[8] = U0[8] + du[8];
+  const bool bad = S->check != 0 && inadmissible(rh, P.smallr, P.pfl, P.gamma, P.llf_dmin);
+  Q<T> U0q;
+  for (int v = 0; v < 9; ++v) U0q.v[v] = U0[v];
+  Q<T> r = S->drag ? drag_correct(U0q, rh, S->q, S->vx0, S->vy0, S->vz0, P.smallr, T(1e-30),
+                                  cr ? &cr[8 * nc + c] : nullptr, S->track_e)
+                   : rh;
+  for (int v = 0; v < 8; ++v) o[v][c] = r.v[v];
+  if (bad) o[0][c] = -std::max(std::fabs(r.v[0]), std::max(P.smallr, P.llf_dmin));
+  o[8][c] = ctp ? T(0) : r.v[8] * decay;
 }
 
 template <class T>
-void step_ref(const T* U, T* O, const Grid& G, T dtdx, T ch, T decay, const Phys<T>& P, T* ctp) {
+void step_ref(const T* U, T* O, const Grid& G, T dtdx, T ch, T decay, const Phys<T>& P, T* ctp,
+              const Src<T>* S = nullptr) {
   const int64_t nc = G.nx * G.ny * G.nz;
   const T* u[9];
   T* o[9];
   for (int v = 0; v < 9; ++v) { u[v] = U + v * nc; o[v] = O + v * nc; }
+  if (S && S->rad) u[8] = S->corr;  // psi enters through slot 0 of the correction buffer
 #pragma omp parallel for schedule(static)
-  for (int64_t c = 0; c < nc; ++c) step_cell(u, o, G, c, dtdx, ch, decay, P, ctp);
+  for (int64_t c = 0; c < nc; ++c) step_cell(u, o, G, c, dtdx, ch, decay, P, ctp, S);
 }
 
 // timestep.jl:4-15: per-cell max_d(|v_d| + c_f(q, gamma, B_d)), then the max.
@@ -582,6 +783,19 @@ static Phys<T> mkphys(double g, double smallr, double pfl, double dmin, double p
     step_ref<T>(U, O, G, T(dtdx), T(ch), T(decay), mkphys<T>(g, smallr, pfl, dmin, pmin, recon, rs),    \
                 ctp);                                                                                   \
   }                                                                                                     \
+  /* flags: 1 drag, 2 radiation, 4/8/16 track rho/B/E, check mode in bits 8-9 */                          \
+  extern "C" void orc_step_src_##SUF(const T* U, T* O, int64_t nx, int64_t ny, int64_t nz, const int* bc,  \
+                                     int recon, int rs, double dtdx, double ch, double decay, double g,    \
+                                     double smallr, double pfl, double dmin, double pmin, T* ctp,           \
+                                     int flags, double q, double vx0, double vy0, double vz0, T* corr) {    \
+    Grid G{nx, ny, nz, {bc[0], bc[1], bc[2]}};                                                           \
+    Src<T> S;                                                                                            \
+    S.drag = flags & 1; S.rad = flags & 2; S.track_rho = flags & 4; S.track_b = flags & 8;               \
+    S.track_e = flags & 16; S.check = (flags >> 8) & 3;                                                  \
+    S.q = T(q); S.vx0 = T(vx0); S.vy0 = T(vy0); S.vz0 = T(vz0); S.corr = corr;                           \
+    step_ref<T>(U, O, G, T(dtdx), T(ch), T(decay), mkphys<T>(g, smallr, pfl, dmin, pmin, recon, rs), ctp, \
+                &S);                                                                                     \
+  }                                                                                                      \
   extern "C" double orc_max_wavespeed_##SUF(const T* U, int64_t nx, int64_t ny, int64_t nz, double g,   \
                                             double smallr, double pfl) {                                \
     Grid G{nx, ny, nz, {0, 0, 0}};                                                                      \
@@ -648,6 +862,22 @@ static Phys<T> mkphys(double g, double smallr, double pfl, double dmin, double p
 
 ORC_DEFINE(f64, double)
 ORC_DEFINE(f32, float)
+
+// Drag propagator coefficients at impulse q: exp(-q), phi1(q), A, C, D (glm_physics.jl:63-103).
+extern "C" void orc_drag_coeffs_f64(double q, double* out) {
+  out[0] = std::exp(-q);
+  out[1] = drag_phi1(q);
+  drag_work(q, out[2], out[3], out[4]);
+}
+extern "C" void orc_drag_coeffs_f32(double q, double* out) {
+  float A, C, D, qf = float(q);
+  out[0] = std::exp(-qf);
+  out[1] = drag_phi1(qf);
+  drag_work(qf, A, C, D);
+  out[2] = A;
+  out[3] = C;
+  out[4] = D;
+}
 
 extern "C" int orc_num_threads(void) {
 #ifdef _OPENMP

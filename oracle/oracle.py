@@ -53,6 +53,8 @@ def lib():
         ci = ctypes.c_int
         for suf in ("f64", "f32"):
             getattr(L, f"orc_step_{suf}").argtypes = [P, P, i64, i64, i64, P, ci, ci, d, d, d, d, d, d, d, d, P]
+            getattr(L, f"orc_step_src_{suf}").argtypes = [P, P, i64, i64, i64, P, ci, ci, d, d, d, d, d, d, d, d, P,
+                                                          ci, d, d, d, d, P]
             getattr(L, f"orc_max_wavespeed_{suf}").argtypes = [P, i64, i64, i64, d, d, d]
             getattr(L, f"orc_max_wavespeed_{suf}").restype = d
             getattr(L, f"orc_max_divb_{suf}").argtypes = [P, i64, i64, i64, P, d]
@@ -137,6 +139,38 @@ class OracleState:
         if self.ct is not None:
             getattr(L, f"orc_ct_update_{_suf(self.dtype)}")(_p(self.U), _p(self.ct), nx, ny, nz, float(dtdx),
                                                              1 if compensate else 0)
+
+    def step_sources(self, dt, ch, decay, drag_impulse=0.0, target_velocity=(0.0, 0.0, 0.0), correction=None,
+                     track_density=False, track_magnetic=False, track_energy=False, check=0, drag=True,
+                     recon=None, riemann=None):
+        """step_ref_drag! (correction None) / step_ref_radiation! (integrator_ref.jl:224-258):
+        kernel -> swap -> ct_update!(compensate=track_magnetic). `correction` is a (9, nc)
+        array in the state's precision, updated in place (slots 4-8 when tracking). `check`
+        follows the cube kernel: 1 flags inadmissible cells with a negative density, 2
+        zeroes the slopes of cells whose slot 4 is positive and solves their faces with LLF,
+        3 zeroes every slope (the caller passes riemann="llf", as step_cube_radiation! does).
+        recon / riemann override the state's codes for this call."""
+        T = self.dtype.type
+        dtdx = T(dt) / self.dx
+        L = lib()
+        nx, ny, nz = self.dims
+        rad = correction is not None
+        if rad:
+            assert correction.dtype == self.dtype and correction.shape == (9, self.nc)
+            assert correction.flags["C_CONTIGUOUS"]
+        flags = (1 if drag else 0) | (2 if rad else 0) | (4 if track_density else 0) | \
+            (8 if track_magnetic else 0) | (16 if track_energy else 0) | (int(check) << 8)
+        getattr(L, f"orc_step_src_{_suf(self.dtype)}")(
+            _p(self.U), _p(self.scratch), nx, ny, nz, self._bc(), RECON[recon or self.recon],
+            RIEMANN[riemann or self.riemann], float(dtdx), float(T(ch)), float(T(decay)), float(self.gamma),
+            float(self.smallr), float(self.pfl), float(self.llf_dmin), float(self.llf_pmin),
+            _p(self.ct) if self.ct is not None else None, flags, float(T(drag_impulse)),
+            float(T(target_velocity[0])), float(T(target_velocity[1])), float(T(target_velocity[2])),
+            _p(correction) if rad else None)
+        self.U, self.scratch = self.scratch, self.U
+        if self.ct is not None:
+            getattr(L, f"orc_ct_update_{_suf(self.dtype)}")(_p(self.U), _p(self.ct), nx, ny, nz, float(dtdx),
+                                                             1 if (rad and track_magnetic) else 0)
 
     def step(self, dt, ch=None, glm_cr=GLM_CR):
         """driver.jl:20-37 (step! with integrator=:ref)."""
@@ -295,6 +329,15 @@ def predict_batch(M, dtdx, dtype=np.float64, recon="plm", gamma=5.0 / 3.0, small
     getattr(lib(), f"orc_predict_batch_{_suf(dtype)}")(_p(M), _p(MH), _p(DL), _p(DR), n, RECON[recon],
                                                        float(dtdx), float(gamma), float(smallr), float(pfl))
     return MH, DL, DR
+
+
+def drag_coefficients(q, dtype=np.float64):
+    """(exp(-q), phi1(q), A, C, D) of the drag propagator (glm_physics.jl:63-103)."""
+    out = np.zeros(5)
+    fn = getattr(lib(), f"orc_drag_coeffs_{_suf(dtype)}")
+    fn.argtypes = [ctypes.c_double, ctypes.c_void_p]
+    fn(float(q), out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 def num_threads() -> int:

@@ -1,0 +1,211 @@
+"""Drag / radiation-coupled steps (step_ref_drag!, step_ref_radiation!, step_cube_drag!,
+step_cube_radiation!; integrator_ref.jl:28-48,150-160,222-258, integrator_cube.jl:68-116,
+478-540, glm_physics.jl:63-170,350-390): pins of the oracle's source terms on the CPU,
+then the GPU kernels against it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_oracle import smooth_state
+
+G = 5.0 / 3.0
+
+
+def _uniform(n, rho=1.3, v=(0.4, -0.2, 0.3), p=0.8, B=(0.5, 0.2, -0.1), psi=0.0):
+    q = np.zeros((9, n ** 3))
+    q[0], q[4] = rho, p
+    q[1:4] = np.array(v)[:, None]
+    q[5:8] = np.array(B)[:, None]
+    q[8] = psi
+    from pmfmhd import problems as P
+    return P.prim2cons(q, G)
+
+
+def _eint(U):
+    return U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0] - 0.5 * (U[5] ** 2 + U[6] ** 2 + U[7] ** 2)
+
+
+def test_drag_coefficients_closed_forms_and_series_branches():
+    for q in (0.3, 1.7, 0.06):
+        e, phi1, A, C, D = O.drag_coefficients(q)
+        em1, em2 = -np.expm1(-q), -np.expm1(-2 * q)
+        assert e == pytest.approx(np.exp(-q), rel=1e-15)
+        assert phi1 == pytest.approx(em1 / q, rel=1e-14)
+        assert A == pytest.approx(0.5 * em2, rel=1e-14)
+        assert C == pytest.approx(em1 * em1 / q, rel=1e-14)
+        assert D == pytest.approx((q - 2 * em1 + 0.5 * em2) / q ** 2, rel=1e-9)
+    # series branches (q < 1e-4 for phi1, q < 0.05 for A, C, D) against the exact forms
+    # evaluated in long double-free ways: the truncation is O(q^3) / O(q^6)
+    q = 9e-5
+    assert O.drag_coefficients(q)[1] == pytest.approx(-np.expm1(-q) / q, rel=1e-12)
+    q = 0.049
+    _, _, A, C, D = O.drag_coefficients(q)
+    em1, em2 = -np.expm1(-q), -np.expm1(-2 * q)
+    assert A == pytest.approx(0.5 * em2, rel=2e-6)
+    assert C == pytest.approx(em1 * em1 / q, rel=2e-6)
+    # D from its Taylor series to high order (the closed form cancels badly here)
+    Dser = q * sum(c * q ** k for k, c in enumerate([1 / 3, -1 / 4, 7 / 60, -1 / 24, 31 / 2520, -1 / 320,
+                                                     127 / 181440]))
+    assert D == pytest.approx(Dser, rel=2e-6)
+
+
+@pytest.mark.parametrize("target", [(0.0, 0.0, 0.0), (0.1, -0.3, 0.05)])
+def test_drag_on_a_uniform_state_is_the_exact_exponential_relaxation(target):
+    """No gradients: fluxes cancel, so the step is the drag propagator alone. Velocity
+    relaxes as v0 + (v - v0) e^{-q}; the energy lost equals the kinetic energy lost (the
+    drag work), so internal energy, density and B are untouched."""
+    n, q = 4, 0.37
+    U = _uniform(n)
+    s = O.OracleState((n, n, n), 1.0 / n, recon="plm_minmod")
+    s.set_conserved(U)
+    s.step_sources(1e-3, 1.0, 1.0, drag_impulse=q, target_velocity=target)
+    v = U[1:4, 0] / U[0, 0]
+    vn = s.U[1:4] / s.U[0]
+    want = np.array(target) + (v - np.array(target)) * np.exp(-q)
+    np.testing.assert_allclose(vn, np.broadcast_to(want[:, None], vn.shape), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(_eint(s.U), _eint(U), rtol=1e-13)
+    np.testing.assert_allclose(s.U[[0, 5, 6, 7]], U[[0, 5, 6, 7]], rtol=1e-14)
+
+
+def test_zero_impulse_drag_is_the_plain_step_and_zero_correction_radiation_is_the_drag_step():
+    n = 8
+    U = smooth_state(n, seed=3)
+    a = O.OracleState((n, n, n), 1.0 / n, recon="plm")
+    a.set_conserved(U)
+    dt, ch = a.compute_dt(0.3)
+    decay = 0.97
+    b, c, d = a.copy(), a.copy(), a.copy()
+    a.step_ref(dt, ch, decay)
+    b.step_sources(dt, ch, decay, drag_impulse=0.0)
+    scale = np.abs(a.U).max(axis=1, keepdims=True)
+    assert (np.abs(b.U - a.U) / scale).max() < 1e-14
+    c.step_sources(dt, ch, decay, drag_impulse=0.2)
+    corr = np.zeros((9, n ** 3))
+    corr[0] = U[8]  # psi enters through slot 0
+    d.step_sources(dt, ch, decay, drag_impulse=0.2, correction=corr)
+    np.testing.assert_array_equal(d.U, c.U)
+    assert np.abs(c.U - a.U).max() > 1e-6
+
+
+def test_radiation_velocity_correction_advects_along_its_own_axis():
+    """A uniform velocity correction c_x on a density ripple at rest in x acts on the face
+    states as an extra velocity: to first order the density moves as d rho = -c dt d_x rho,
+    and a correction put on the y slot leaves this x-only ripple alone."""
+    n, c, amp = 32, 0.05, 1e-3
+    from pmfmhd import problems as P
+    i = np.arange(n ** 3) % n
+    x = 2 * np.pi * (i + 0.5) / n
+    q = np.zeros((9, n ** 3))
+    q[0] = 1 + amp * np.sin(x)
+    q[4] = 1.0
+    U = P.prim2cons(q, G)
+    s = O.OracleState((n, n, n), 1.0 / n, recon="plm")
+    s.set_conserved(U)
+    dt = 0.2 / n
+    for slot, moves in ((1, True), (2, False)):
+        t = s.copy()
+        corr = np.zeros((9, n ** 3))
+        corr[slot] = c
+        t.step_sources(dt, 1.0, 1.0, drag_impulse=0.0, correction=corr)
+        drho = t.U[0] - U[0]
+        want = -c * dt * amp * np.cos(x) * 2 * np.pi
+        if moves:
+            assert np.linalg.norm(drho - want) < 0.2 * np.linalg.norm(want)
+        else:
+            assert np.abs(drho).max() < 1e-3 * np.abs(want).max()
+
+
+def test_tracking_carries_the_rounding_of_each_update():
+    """Float32 with tracking: slot 4 accumulates the density increments (the result itself
+    is unchanged), and the energy / B residual slots hold exactly the rounding error of
+    each compensated update, so value + residual equals the float64 sum of the inputs."""
+    n = 8
+    U = smooth_state(n, seed=5).astype(np.float32)
+    s = O.OracleState((n, n, n), 1.0 / n, dtype=np.float32, recon="plm")
+    s.set_conserved(U)
+    dt, ch = s.compute_dt(0.3)
+    corr = np.zeros((9, n ** 3), dtype=np.float32)
+    corr[0] = U[8]
+    corr[5:9] = np.float32(1e-9)  # carried residuals from an earlier step
+    plain = s.copy()
+    plain.step_sources(dt, ch, 1.0, drag_impulse=0.0, correction=np.ascontiguousarray(corr.copy()))
+    c0 = corr.copy()
+    s.step_sources(dt, ch, 1.0, drag_impulse=0.0, correction=corr, track_density=True, track_magnetic=True,
+                   track_energy=True)
+    np.testing.assert_array_equal(s.U[0], plain.U[0])
+    np.testing.assert_allclose(corr[4], (s.U[0].astype(np.float64) - U[0]), rtol=0, atol=2e-7)
+    # exact Fast2Sum invariant for B: out + res_out = U0 + (du + res_in) with du recovered
+    # from the untracked step (same fluxes): out_plain = U0 + du
+    for v, slot in ((5, 5), (6, 6), (7, 7)):
+        du = plain.U[v].astype(np.float64) - U[v]  # exact up to the plain rounding
+        lhs = s.U[v].astype(np.float64) + corr[slot]
+        rhs = U[v].astype(np.float64) + du + c0[slot]
+        assert np.abs(lhs - rhs).max() < 4e-7 * np.abs(U[v]).max()
+        assert np.abs(corr[slot]).max() < 1e-6  # a rounding error, not an increment
+
+
+def test_admissibility_flag_marks_only_the_broken_cell():
+    n = 8
+    U = smooth_state(n, seed=2)
+    c = 3 * n * n + 2 * n + 5
+    U[4, c] = 0.5 * (U[1, c] ** 2 + U[2, c] ** 2 + U[3, c] ** 2) / U[0, c] + \
+        0.5 * (U[5, c] ** 2 + U[6, c] ** 2 + U[7, c] ** 2) - 0.4  # negative internal energy
+    a = O.OracleState((n, n, n), 1.0 / n, recon="plm_minmod", riemann="hll")
+    a.set_conserved(U)
+    b = a.copy()
+    dt, ch = 1e-3, 2.0
+    corr = np.zeros((9, n ** 3))
+    corr[0] = U[8]
+    a.step_sources(dt, ch, 1.0, drag_impulse=0.1, correction=corr.copy())
+    b.step_sources(dt, ch, 1.0, drag_impulse=0.1, correction=corr.copy(), check=1)
+    bad = b.U[0] < 0
+    assert bad[c] and bad.sum() >= 1
+    np.testing.assert_array_equal(b.U[:, ~bad], a.U[:, ~bad])
+    np.testing.assert_allclose(-b.U[0, bad], np.maximum(np.abs(a.U[0, bad]), 1e-4))
+
+
+def test_global_fallback_equals_masked_fallback_with_every_cell_flagged():
+    n = 8
+    U = smooth_state(n, seed=4)
+    a = O.OracleState((n, n, n), 1.0 / n, recon="plm", riemann="hlld")
+    a.set_conserved(U)
+    b = a.copy()
+    dt, ch = a.compute_dt(0.3)
+    corr = np.zeros((9, n ** 3))
+    corr[0] = U[8]
+    ca, cb = corr.copy(), corr.copy()
+    cb[4] = 1.0
+    a.step_sources(dt, ch, 0.9, drag_impulse=0.05, correction=ca, check=3, riemann="llf")
+    b.step_sources(dt, ch, 0.9, drag_impulse=0.05, correction=cb, check=2)
+    np.testing.assert_array_equal(a.U, b.U)
+    # and a flagged uniform block stays first order: zero slopes give the cell-mean faces
+    c = O.OracleState((n, n, n), 1.0 / n, recon="plm")
+    c.set_conserved(_uniform(n))
+    cc = np.zeros((9, n ** 3))
+    cc[4] = 1.0
+    c.step_sources(dt, ch, 1.0, drag_impulse=0.0, correction=cc, check=2)
+    np.testing.assert_allclose(c.U, _uniform(n), rtol=1e-14, atol=1e-15)
+
+
+def test_drag_propagates_the_face_velocity_to_the_half_step():
+    """A density ripple advected by a uniform velocity u: with a drag impulse q the faces
+    carry the half-step velocity u e^{-q/2} (the propagated face prediction), so the
+    density change is the plain one scaled by e^{-q/2} (0.61 at q = 1; phi1(q/2) = 0.79 or
+    no propagation = 1 would be far off)."""
+    n, u, amp, q = 32, 0.5, 1e-3, 1.0
+    from pmfmhd import problems as P
+    i = np.arange(n ** 3) % n
+    x = 2 * np.pi * (i + 0.5) / n
+    qp = np.zeros((9, n ** 3))
+    qp[0] = 1 + amp * np.sin(x)
+    qp[1] = u
+    qp[4] = 1.0
+    U = P.prim2cons(qp, G)
+    s = O.OracleState((n, n, n), 1.0 / n, recon="plm")
+    s.set_conserved(U)
+    dt = 0.1 / n
+    a, b = s.copy(), s.copy()
+    a.step_sources(dt, 1.0, 1.0, drag_impulse=0.0)
+    b.step_sources(dt, 1.0, 1.0, drag_impulse=q)
+    ratio = np.linalg.norm(b.U[0] - U[0]) / np.linalg.norm(a.U[0] - U[0])
+    assert ratio == pytest.approx(np.exp(-q / 2), rel=0.03)
