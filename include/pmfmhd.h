@@ -170,6 +170,42 @@ int pmf_surrogate_face_flux(pmf_surrogate* n, int64_t first, int64_t count, int3
 int pmf_step_surrogate(pmf_state* s, pmf_surrogate* n, double dt, double ch, double glm_cr, double thr,
                        int64_t* nsmooth);
 
+/* ---- Drag / radiation-coupled steps -----------------------------------------------
+ * step_ref_drag! / step_cube_drag! (integrator_ref.jl:222-236, integrator_cube.jl:478-503):
+ * the step with the Compton-drag propagator of impulse q = Gamma dt toward the target
+ * velocity (NULL = rest; the :cube integrator takes rest only). Swaps like pmf_step. */
+int pmf_step_drag(pmf_state* s, double dt, double ch, double decay, double drag_impulse,
+                  const double* target_velocity, int32_t integrator);
+/* step_ref_radiation! / step_cube_radiation! (integrator_ref.jl:238-258,
+ * integrator_cube.jl:505-540): drag plus the radiation correction buffer, a device array of
+ * 9 slots of ncells in the state's precision: psi (read instead of the state's psi), the
+ * three velocity corrections added to every face state of a cell, then the density
+ * perturbation (or fallback mask), three field and one energy residual updated in place
+ * when tracked. flags: PMF_SRC_*; the check and fallback options are :cube only. */
+enum {
+  PMF_SRC_TRACK_DENSITY = 1,
+  PMF_SRC_TRACK_MAGNETIC = 2,
+  PMF_SRC_TRACK_ENERGY = 4,
+  PMF_SRC_CHECK_ADMISSIBILITY = 8, /* inadmissible cells come back with a negative density */
+  PMF_SRC_FALLBACK = 16,           /* cells with slot 4 > 0: zero slopes, LLF on their faces */
+  PMF_SRC_GLOBAL_FALLBACK = 32,    /* every cell first order, LLF everywhere */
+  PMF_SRC_ROBUST_FALLBACK = 64     /* minmod PLM and HLL for this step */
+};
+int pmf_step_radiation(pmf_state* s, double dt, double ch, double decay, double drag_impulse, void* correction,
+                       int32_t flags, int32_t integrator);
+/* The _cube_drag_launch! / _cube_radiation_launch! seam: one coupled step from nine
+ * caller arrays into nine others (no swap); radiation when correction != NULL. */
+int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, void* ct, void* correction, double dt,
+                          double ch, double decay, double drag_impulse, const double* target_velocity,
+                          int32_t flags, int32_t integrator);
+
+/* Device buffers for callers without a GPU array library (e.g. the radiation
+ * correction buffer): zero-filled allocation, free, and copies (kind 1 host to device,
+ * 2 device to host, 3 device to device; the copies wait for the device). */
+int pmf_buffer_alloc(int64_t bytes, void** out);
+int pmf_buffer_free(void* p);
+int pmf_buffer_copy(void* dst, const void* src, int64_t bytes, int32_t kind);
+
 /* ---- Ensembles ------------------------------------------------------------------
  * No counterpart in the reference, which advances one MHDState per step! call
  * (driver.jl:20-37): many independent periodic float32 GLM cubes of uneven sizes

@@ -76,6 +76,126 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
   }
 }
 
+// The drag / radiation-coupled steps (step_ref_drag!, step_ref_radiation!,
+// step_cube_drag!, step_cube_radiation!: integrator_ref.jl:222-258,
+// integrator_cube.jl:478-540).
+struct SourceCall {
+  bool radiation = false;
+  double impulse = 0.0;
+  double v0[3] = {0.0, 0.0, 0.0};
+  void* corr = nullptr;
+  int flags = 0;  // PMF_SRC_* bits
+};
+
+// glm_physics.jl:63-65,87-103,118-122, evaluated in the state's precision.
+template <class T> T drag_phi1(T q) {
+  return std::fabs(q) < T(1e-4) ? T(1) - q / T(2) + q * q / T(6) : -std::expm1(-q) / q;
+}
+template <class T> void drag_work(T q, T& A, T& C, T& D) {
+  if (q < T(0.05)) {
+    T q2 = q * q, q3 = q2 * q, q4 = q3 * q;
+    A = q * (T(1) - q + T(2.0 / 3.0) * q2 - T(1.0 / 3.0) * q3 + T(2.0 / 15.0) * q4);
+    C = q * (T(1) - q + T(7.0 / 12.0) * q2 - T(1.0 / 4.0) * q3 + T(31.0 / 360.0) * q4);
+    D = q * (T(1.0 / 3.0) - q / T(4) + T(7.0 / 60.0) * q2 - q3 / T(24) + T(31.0 / 2520.0) * q4);
+    return;
+  }
+  T em1 = -std::expm1(-q), em2 = -std::expm1(-T(2) * q);
+  A = T(0.5) * em2;
+  C = em1 * em1 / q;
+  D = (q - T(2) * em1 + T(0.5) * em2) / (q * q);
+}
+
+template <class T>
+void step_fields_src(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double dt, double ch, double decay,
+                     int integrator, const SourceCall& a) {
+  const bool ct = s->p.solver == PMF_SOLVER_CT;
+  const bool cube = integrator == PMF_INTEGRATOR_CUBE;
+  const char* what = a.radiation ? (cube ? "step_cube_radiation!" : "step_ref_radiation!")
+                                 : (cube ? "step_cube_drag!" : "step_ref_drag!");
+  if (integrator != PMF_INTEGRATOR_REF && !cube)
+    throw Error("unknown integrator :" + std::to_string(integrator) + " (have :ref, :cube)");
+  if (ct && cube)
+    throw Error(std::string(what) + ": CT is implemented on the :ref integrator only (mhd_solver=:ct states cannot "
+                                    "use the :cube integrators)");
+  if (cube && (a.v0[0] != 0.0 || a.v0[1] != 0.0 || a.v0[2] != 0.0))
+    throw Error("step_cube_drag! is specialized for Compton drag in the CMB frame; use integrator=:ref for a moving "
+                "target");
+  const int checks = PMF_SRC_CHECK_ADMISSIBILITY | PMF_SRC_FALLBACK | PMF_SRC_GLOBAL_FALLBACK;
+  if (!cube && (a.flags & checks))
+    throw Error(std::string(what) + ": the admissibility check and the fallback masks are options of the :cube "
+                                    "integrator");
+  if ((a.flags & PMF_SRC_FALLBACK) && (a.flags & PMF_SRC_TRACK_DENSITY))
+    throw Error(std::string(what) + ": the masked fallback reads correction slot 4, which density tracking writes");
+  if ((a.radiation || (a.flags & PMF_SRC_FALLBACK)) && !a.corr)
+    throw Error(std::string(what) + " needs the correction buffer (9 slots of ncells)");
+  if (!ct && !(ch > 0.0)) throw Error("step! needs the GLM cleaning speed `ch` (GLM states)");
+  if (ct && !ctp) throw Error("ct_update! needs a CT state");
+  SrcParams<T> S{};
+  const T q = T(a.impulse), qh = T(0.5) * q;
+  S.face_decay = std::exp(-qh);
+  S.face_phi1 = drag_phi1(qh);
+  S.decay = std::exp(-q);
+  S.phi1 = drag_phi1(q);
+  drag_work(q, S.A, S.C, S.D);
+  S.vx0 = T(a.v0[0]);
+  S.vy0 = T(a.v0[1]);
+  S.vz0 = T(a.v0[2]);
+  S.corr = (T*)a.corr;
+  S.stride = s->nc;
+  S.flags = SRC_DRAG;
+  if (a.radiation) {
+    S.flags |= SRC_RAD;
+    if (a.flags & PMF_SRC_TRACK_DENSITY) S.flags |= SRC_TRACK_RHO;
+    if (a.flags & PMF_SRC_TRACK_MAGNETIC) S.flags |= SRC_TRACK_B;
+    if (a.flags & PMF_SRC_TRACK_ENERGY) S.flags |= SRC_TRACK_E;
+  }
+  S.check = (a.flags & PMF_SRC_GLOBAL_FALLBACK)     ? CHECK_GLOBAL
+            : (a.flags & PMF_SRC_FALLBACK)          ? CHECK_MASKED
+            : (a.flags & PMF_SRC_CHECK_ADMISSIBILITY) ? CHECK_FLAG
+                                                      : CHECK_NONE;
+  // psi enters through slot 0 of the correction buffer (integrator_ref.jl:242-243)
+  if (a.radiation) in.v[8] = (const T*)a.corr;
+  StepParams<T> P = make_params<T>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay);
+  if (a.flags & PMF_SRC_ROBUST_FALLBACK) {  // the minmod-PLM / HLL trial (integrator_cube.jl:523-525)
+    P.recon = RECON_PLM_MINMOD;
+    P.rsolve = RS_HLL;
+  } else if (a.flags & PMF_SRC_GLOBAL_FALLBACK) {
+    P.rsolve = RS_LLF;
+  }
+  if (cube) {
+    if (!fused_supported(s->G, s->p.precision, P.recon, P.rsolve))
+      throw Error(std::string(what) + ": the fused kernel needs an all-periodic grid whose nx is a multiple of 16 "
+                                      "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" +
+                  std::to_string(s->p.nz) + ")");
+    mark(s);
+    bool done = launch_step_fused<T>(in, out, s->G, P, s->red, s->dval, s->st, s->zc, &S);
+    mark(s);
+    if (!done) throw Error(std::string(what) + ": fused kernel unavailable for this configuration");
+  } else {
+    mark(s);
+    launch_step_ref<T>(in, out, ctp, s->G, P, ct, s->st, &S);
+    mark(s);
+  }
+  hip_check(hipGetLastError(), "step launch");
+  if (ct) {
+    // compensated faces exactly when GLM would compensate cell B (integrator_ref.jl:255-256)
+    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), a.radiation && (a.flags & PMF_SRC_TRACK_MAGNETIC), s->st);
+    hip_check(hipGetLastError(), "ct_update launch");
+  }
+}
+
+void do_step_src(pmf_state* s, double dt, double ch, double decay, int integrator, const SourceCall& a) {
+  check_fields(s);
+  if (s->p.precision == PMF_F64)
+    step_fields_src<double>(s, cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc), (double*)s->ct, dt,
+                            ch, decay, integrator, a);
+  else
+    step_fields_src<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), (float*)s->ct, dt, ch,
+                           decay, integrator, a);
+  std::swap(s->U, s->S);
+  s->smax_valid = integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM;
+}
+
 // step_ref! / step_cube! on the state's own fields: into scratch, then swap.
 void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
   check_fields(s);
@@ -577,6 +697,78 @@ int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax) {
     hip_check(hipGetLastError(), "wavespeed launch");
     fetch(s, 0, 1);
     *smax = s->hval[0];
+  });
+}
+
+int pmf_step_drag(pmf_state* s, double dt, double ch, double decay, double drag_impulse,
+                  const double* target_velocity, int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    SourceCall a;
+    a.impulse = drag_impulse;
+    if (target_velocity)
+      for (int d = 0; d < 3; ++d) a.v0[d] = target_velocity[d];
+    do_step_src(s, dt, ch, decay, integrator, a);
+  });
+}
+
+int pmf_step_radiation(pmf_state* s, double dt, double ch, double decay, double drag_impulse, void* correction,
+                       int32_t flags, int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    SourceCall a;
+    a.radiation = true;
+    a.impulse = drag_impulse;
+    a.corr = correction;
+    a.flags = flags;
+    do_step_src(s, dt, ch, decay, integrator, a);
+  });
+}
+
+int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, void* ct, void* correction, double dt,
+                          double ch, double decay, double drag_impulse, const double* target_velocity,
+                          int32_t flags, int32_t integrator) {
+  return guard([&] {
+    check_state(s);
+    if (!U || !O) throw Error("null field array");
+    SourceCall a;
+    a.radiation = correction != nullptr;
+    a.impulse = drag_impulse;
+    a.corr = correction;
+    a.flags = flags;
+    if (target_velocity)
+      for (int d = 0; d < 3; ++d) a.v0[d] = target_velocity[d];
+    if (s->p.precision == PMF_F64)
+      step_fields_src<double>(s, cfields_from<double>(U), fields_from<double>(O), (double*)ct, dt, ch, decay,
+                              integrator, a);
+    else
+      step_fields_src<float>(s, cfields_from<float>(U), fields_from<float>(O), (float*)ct, dt, ch, decay, integrator,
+                             a);
+  });
+}
+
+int pmf_buffer_alloc(int64_t bytes, void** out) {
+  return guard([&] {
+    if (!out || bytes <= 0) throw Error("bad buffer request");
+    *out = nullptr;
+    hip_check(hipMalloc(out, size_t(bytes)), "hipMalloc");
+    hip_check(hipMemset(*out, 0, size_t(bytes)), "hipMemset");
+  });
+}
+
+int pmf_buffer_free(void* p) {
+  return guard([&] {
+    if (p) hip_check(hipFree(p), "hipFree");
+  });
+}
+
+int pmf_buffer_copy(void* dst, const void* src, int64_t bytes, int32_t kind) {
+  return guard([&] {
+    if (!dst || !src || bytes < 0) throw Error("bad copy request");
+    if (kind < 1 || kind > 3) throw Error("kind: 1 host to device, 2 device to host, 3 device to device");
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hip_check(hipMemcpy(dst, src, size_t(bytes), k), "hipMemcpy");
   });
 }
 

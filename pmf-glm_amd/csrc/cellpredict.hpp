@@ -53,7 +53,7 @@ __device__ __forceinline__ void offsets9(const T* qm, const T* q0, const T* qp, 
 // per-axis face offsets.
 template <int RECON, class T>
 __device__ void predict_cell(const Fields<const T>& U, const Grid& G, int i, int j, int k, const StepParams<T>& P,
-                             T* mh, T (*dL)[9], T (*dR)[9]) {
+                             T* mh, T (*dL)[9], T (*dR)[9], T* m0out = nullptr) {
   T m0[9], qm[9], qp[9];
   load_prim(U, G, i, j, k, m0, P);
   load_prim(U, G, i - 1, j, k, qm, P);
@@ -96,6 +96,20 @@ __device__ void predict_cell(const Fields<const T>& U, const Grid& G, int i, int
 #pragma unroll
   for (int v = 0; v < 9; ++v) u0[v] -= P.hdtdx * acc[v];
   cons2prim(u0, mh, P);
+  if (m0out) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) m0out[v] = m0[v];
+  }
+}
+
+// Periodic linear index of logical cell (i,j,k): the radiation correction slots are
+// addressed this way whatever the boundary conditions (integrator_ref.jl:50-53).
+__device__ __forceinline__ int64_t periodic_index(const Grid& G, int i, int j, int k) {
+  int ii = i % G.nx, jj = j % G.ny, kk = k % G.nz;
+  ii += ii < 0 ? G.nx : 0;
+  jj += jj < 0 ? G.ny : 0;
+  kk += kk < 0 ? G.nz : 0;
+  return ((int64_t)kk * G.ny + jj) * G.nx + ii;
 }
 
 }  // namespace pmf

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "mhd_device.hpp"
+#include "sources.hpp"
 
 namespace pmf {
 
@@ -11,17 +12,19 @@ namespace pmf {
 constexpr int kRedBlocks = 2048;
 constexpr int kRedSlots = 24;  // doubles per block slot
 
+// S: drag / radiation source terms (nullptr for the plain step).
 template <class T>
 void launch_step_ref(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P, bool ct,
-                     hipStream_t st);
+                     hipStream_t st, const SrcParams<T>* S = nullptr);
 
 // Fused tiled step (periodic GLM). Writes per-block CFL maxima of the new state into
 // red (one double per block) and the final max into dsmax. Returns false when the grid
 // shape is not supported by the fused kernel.
-// zc: planes per z chunk (0 = the built-in choice).
+// zc: planes per z chunk (0 = the built-in choice); src: drag / radiation source terms
+// (nullptr for the plain step).
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st, int zc = 0);
+                       double* dsmax, hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
 // The surrogate-assisted fused step (float32, HLLD): simg is the network's LDS image,
 // nsmooth accumulates the number of faces solved by the network.

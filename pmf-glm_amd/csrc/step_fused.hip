@@ -98,6 +98,7 @@ struct FusedLds {
   T sP[2][9 * FNP];  // primitive planes m (current) and m-1, alternating
   T sLX[9 * FNQ];    // +x face states, then x-face fluxes
   T sLY[9 * FNQ];    // +y face states, then y-face fluxes
+  unsigned char sMk[FNQ];  // masked-fallback flags of the predicted cells (sources only)
 };
 
 // One work item: tile w (x fastest, then y, then z chunk) of the grid g, marched through
@@ -107,11 +108,11 @@ struct FusedLds {
 // learned flux (wave-level MFMA) or by HLLD; sW/sB hold the network's LDS image
 // (surrogate.hpp) and smooth_count counts the faces that used it (each face once, as
 // the lower face of its cell).
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
-                                        float thr, unsigned& smooth_count) {
+                                        float thr, unsigned& smooth_count, const SrcParams<T>& src) {
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -166,8 +167,19 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       T o[9];
       const int64_t idx = (int64_t)(m - 1) * g.plane + col;
       load_cons(U, idx, g.nc, o);
+      if constexpr (SRC) {
+        // tracked increments, drag propagator, admissibility mark (integrator_cube.jl:423-441)
+        T du[9], U0[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+        for (int v = 0; v < 9; ++v) {
+          U0[v] = o[v];
+          du[v] = P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+        }
+        src_update(U0, du, o, src, idx, P, false);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+      }
       o[8] *= P.decay;
 #pragma unroll
       for (int v = 0; v < 9; ++v) O.v[v][idx] = o[v];
@@ -181,6 +193,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     for (int v = 0; v < 9; ++v) Lzp[v] = Lzn[v];
   };
 
+  static_assert(!(SRC && SURR), "the surrogate path has no source terms");
+  bool mkz = false, mkzp = false;  // masked-fallback flags of this column's planes m and m-1
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
@@ -216,6 +230,20 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         offsets<RECON>(nxt[v * FNP + me], m0[v], Pn[v], dLz[v], dRz[v]);
         nxt[v * FNP + me] = Pn[v];
       }
+      int64_t pcell = 0;  // the predicted cell (plane m of this column)
+      if constexpr (SRC) {
+        // masked / global fallback: flagged cells are predicted with zero slopes
+        // (integrator_cube.jl:330-335)
+        pcell = (int64_t)wrapi(m, g.nz) * g.plane + col;
+        const bool mk = src.check == CHECK_GLOBAL ||
+                        (src.check == CHECK_MASKED && src.corr[4 * src.stride + pcell] > T(0));
+        if (mk) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) dLx[v] = dRx[v] = dLy[v] = dRy[v] = dLz[v] = dRz[v] = T(0);
+        }
+        S.sMk[qidx(ix, iy)] = mk;
+        mkz = mk;
+      }
       T acc[9], fp[9], fm[9], s[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRx[v];
@@ -248,25 +276,55 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       cons2prim(uh, mh, P);
       // face states (integrator_ref.jl:27,74-82): the +d edge is the left state of the
       // upper face, the -d edge the right state of the lower face
+      if constexpr (SRC) {
+        // face states with the drag propagation and the radiation correction
+        // (integrator_cube.jl:342-374, integrator_ref.jl:41-48)
+        T fx[9], fy[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        S.sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
-        S.sLY[v * FNQ + qidx(ix, iy)] = mh[v] + dRy[v];
-        Rx[v] = mh[v] + dLx[v];
-        Ry[v] = mh[v] + dLy[v];
-        Rz[v] = mh[v] + dLz[v];
+        for (int v = 0; v < 9; ++v) {
+          fx[v] = mh[v] + dRx[v];
+          fy[v] = mh[v] + dRy[v];
+          Rx[v] = mh[v] + dLx[v];
+          Ry[v] = mh[v] + dLy[v];
+          Rz[v] = mh[v] + dLz[v];
+          Lzn[v] = mh[v] + dRz[v];
+        }
+        src_face(fx, m0, dRx, src, pcell);
+        src_face(fy, m0, dRy, src, pcell);
+        src_face(Rx, m0, dLx, src, pcell);
+        src_face(Ry, m0, dLy, src, pcell);
+        src_face(Rz, m0, dLz, src, pcell);
+        src_face(Lzn, m0, dRz, src, pcell);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+          S.sLX[v * FNQ + qidx(ix, iy)] = fx[v];
+          S.sLY[v * FNQ + qidx(ix, iy)] = fy[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+          S.sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
+          S.sLY[v * FNQ + qidx(ix, iy)] = mh[v] + dRy[v];
+          Rx[v] = mh[v] + dLx[v];
+          Ry[v] = mh[v] + dLy[v];
+          Rz[v] = mh[v] + dLz[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) Lzn[v] = mh[v] + dRz[v];
       }
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
       // whose inputs all exist now (integrator_ref.jl:140-158)
-#pragma unroll
-      for (int v = 0; v < 9; ++v) Lzn[v] = mh[v] + dRz[v];
       if constexpr (!SURR) {
         if (owned) {
           T Fz[9];
-          if (!first) face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+          if (!first) {
+            if (SRC && src.check == CHECK_MASKED && (mkz || mkzp)) face_flux<2, false, T, RS_LLF>(Lzp, Rz, Fz, P);
+            else face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+          }
           zface_update(Fz, Lzn, m, first);
         }
       }
+      mkzp = mkz;
     }
     else if (has_col) {
       // ring column with no prediction on this plane: still deposit plane m+1
@@ -312,17 +370,24 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         }
         smooth_count += surr_face<1>(L, Rr, Fy, P, thr, ysolver, sW, sB) && stores;
       } else {
+      // masked fallback: LLF on every face touching a flagged cell (integrator_cube.jl:68-80)
       if (xsolver) {
         T L[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
-        face_flux<0, false, T, RS>(L, Rx, Fx, P);
+        if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)]))
+          face_flux<0, false, T, RS_LLF>(L, Rx, Fx, P);
+        else
+          face_flux<0, false, T, RS>(L, Rx, Fx, P);
       }
       if (ysolver) {
         T L[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
-        face_flux<1, false, T, RS>(L, Ry, Fy, P);
+        if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)]))
+          face_flux<1, false, T, RS_LLF>(L, Ry, Fy, P);
+        else
+          face_flux<1, false, T, RS>(L, Ry, Fy, P);
       }
       }
       __syncthreads();
@@ -366,18 +431,18 @@ __device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
 // Three waves per SIMD for the float variants whose registers fit 168 VGPRs without
 // spilling (three 41 KB workgroups then share a CU's LDS); the compiler's default
 // (two waves) for the rest.
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC>
 constexpr bool fused_three_waves() {
-  return sizeof(T) == 4 && !SURR && (RECON == RECON_PLM_MINMOD || (RECON == RECON_PLM && RS != RS_HLLD));
+  return sizeof(T) == 4 && !SURR && !SRC && (RECON == RECON_PLM_MINMOD || (RECON == RECON_PLM && RS != RS_HLLD));
 }
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
 // global memory (SURR only) and nsmooth the count of network-solved faces.
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC>
 __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                                 const StepParams<T>& P, double* __restrict__ red,
                                                 const unsigned char* __restrict__ simg, float thr,
-                                                unsigned long long* __restrict__ nsmooth) {
+                                                unsigned long long* __restrict__ nsmooth, const SrcParams<T>& src) {
   __shared__ FusedLds<T> lds;
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
@@ -395,7 +460,7 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
-  const T smax = fused_item<T, RECON, RS, SURR>(U, O, g, P, w, lds, sW, sB, thr, smooth_count);
+  const T smax = fused_item<T, RECON, RS, SURR, SRC>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src);
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
   block_store_max(smax, sMax, red + blockIdx.x);
   if constexpr (SURR) {
@@ -405,31 +470,31 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   }
 }
 
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
                                                          StepParams<T> P, double* __restrict__ red,
                                                          const unsigned char* __restrict__ simg, float thr,
-                                                         unsigned long long* __restrict__ nsmooth) {
-  step_fused_body<T, RECON, RS, SURR>(U, O, g, P, red, simg, thr, nsmooth);
+                                                         unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+  step_fused_body<T, RECON, RS, SURR, SRC>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused3(
     Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
-    const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth) {
-  step_fused_body<T, RECON, RS, SURR>(U, O, g, P, red, simg, thr, nsmooth);
+    const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+  step_fused_body<T, RECON, RS, SURR, SRC>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false>
 void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                           const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
-                          unsigned long long* nsmooth) {
-  if constexpr (fused_three_waves<T, RECON, RS, SURR>())
-    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg, thr,
-                       nsmooth);
+                          unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{}) {
+  if constexpr (fused_three_waves<T, RECON, RS, SURR, SRC>())
+    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg,
+                       thr, nsmooth, src);
   else
-    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg, thr,
-                       nsmooth);
+    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg,
+                       thr, nsmooth, src);
 }
 
 __global__ __launch_bounds__(256) void k_fold_max_flat(const double* __restrict__ red, int nb, double* __restrict__ out) {
@@ -484,12 +549,14 @@ int64_t fused_blocks(const Grid& G, int zc) {
 
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st, int zc) {
+                       double* dsmax, hipStream_t st, int zc, const SrcParams<T>* src) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
-#define PMF_FUSED(R, S) launch_fused_variant<T, R, S, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
+#define PMF_FUSED(R, S)                                                                                          \
+  if (src) launch_fused_variant<T, R, S, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src);    \
+  else launch_fused_variant<T, R, S, false, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
   else if (P.rsolve == RS_LLF) PMF_FUSED(R, RS_LLF);     \
@@ -637,7 +704,8 @@ __device__ __forceinline__ void ens_step_body(const EnsCube* __restrict__ cubes,
 #pragma unroll
     for (int v = 0; v < 9; ++v) U.v[v] = in.v[v];
     const StepParams<float> P = params[c];
-    const float smax = fused_item<float, RECON, RS, false>(U, out, cb.g, P, w, lds, nullptr, nullptr, 0.f, none);
+    const float smax =
+        fused_item<float, RECON, RS, false>(U, out, cb.g, P, w, lds, nullptr, nullptr, 0.f, none, SrcParams<float>{});
     block_store_max(smax, sMax, red + i);
     __syncthreads();  // sItem and sMax are reused by the next item
   }
@@ -681,7 +749,7 @@ void launch_ens_params(const EnsCube* cubes, int ncubes, const double* red, cons
 void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q, const StepParams<float>* params,
                      double* red, unsigned* ctr, int nwg, int parity, int recon, int rsolve, hipStream_t st) {
 #define PMF_ENS(R, S)                                                                                     \
-  if constexpr (fused_three_waves<float, R, S, false>())                                                     \
+  if constexpr (fused_three_waves<float, R, S, false, false>())                                                     \
     hipLaunchKernelGGL((k_ens_step3<R, S>), dim3(nwg), dim3(FTHREADS), 0, st, cubes, items, q, params, red, ctr, \
                        parity);                                                                              \
   else                                                                                                       \
@@ -702,8 +770,8 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 }
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
-                                       double*, double*, hipStream_t, int);
+                                       double*, double*, hipStream_t, int, const SrcParams<float>*);
 template bool launch_step_fused<double>(Fields<const double>, Fields<double>, const Grid&, const StepParams<double>&,
-                                        double*, double*, hipStream_t, int);
+                                        double*, double*, hipStream_t, int, const SrcParams<double>*);
 
 }  // namespace pmf

@@ -228,6 +228,82 @@ def max_wavespeed(s: MHDState):
     return s.dtype.type(v.value)
 
 
+class DeviceBuffer:
+    """A zero-filled device allocation owned by the library (pmf_buffer_*)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        h = ctypes.c_void_p()
+        check(lib().pmf_buffer_alloc(self.nbytes, ctypes.byref(h)))
+        self._p = h
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def upload(self, a, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise MHDError("upload past the end of the buffer")
+        check(lib().pmf_buffer_copy(ctypes.c_void_p(self.ptr + offset), a.ctypes.data_as(ctypes.c_void_p),
+                                    a.nbytes, 1))
+
+    def download(self, dtype, shape, offset: int = 0):
+        out = np.empty(shape, dtype=dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise MHDError("download past the end of the buffer")
+        check(lib().pmf_buffer_copy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr + offset),
+                                    out.nbytes, 2))
+        return out
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            try:
+                lib().pmf_buffer_free(p)
+            except Exception:
+                pass
+            self._p = None
+
+
+def correction_buffer(s: MHDState, host=None) -> DeviceBuffer:
+    """The radiation correction buffer of a state: 9 slots of ncells in the state's
+    precision (psi, dv_x, dv_y, dv_z, rho perturbation or fallback mask, B_x, B_y, B_z and
+    E residuals), zero-filled or uploaded from a (9, ncells) host array."""
+    b = DeviceBuffer(9 * s.ncells * s.dtype.itemsize)
+    if host is not None:
+        b.upload(np.asarray(host, dtype=s.dtype).reshape(9, s.ncells))
+    return b
+
+
+def load_psi(s: MHDState, corr: DeviceBuffer):
+    """Copy the state's psi into slot 0 of the correction buffer (what the radiation
+    driver does before each coupled step, radiation_coupling.jl:1681)."""
+    p = s.device_ptr(8)
+    check(lib().pmf_buffer_copy(ctypes.c_void_p(corr.ptr), ctypes.c_void_p(p), s.ncells * s.dtype.itemsize, 3))
+
+
+def step_drag(s: MHDState, dt, ch, decay, drag_impulse, target_velocity=(0.0, 0.0, 0.0), integrator="ref"):
+    """step_ref_drag! / step_cube_drag! (integrator_ref.jl:222-236, integrator_cube.jl:478-503)."""
+    v0 = (ctypes.c_double * 3)(*[float(v) for v in target_velocity])
+    check(lib().pmf_step_drag(s.handle, float(dt), float(ch), float(decay), float(drag_impulse), v0,
+                              _integ(integrator)))
+    return s
+
+
+def step_radiation(s: MHDState, dt, ch, decay, drag_impulse, correction: DeviceBuffer, track_density=True,
+                   track_magnetic=True, track_energy=True, check_admissibility=False, fallback=False,
+                   global_fallback=False, robust_fallback=False, integrator="ref"):
+    """step_ref_radiation! / step_cube_radiation! (integrator_ref.jl:238-258,
+    integrator_cube.jl:505-540). psi is read from slot 0 of the correction buffer."""
+    flags = (1 if track_density else 0) | (2 if track_magnetic else 0) | (4 if track_energy else 0) | \
+        (8 if check_admissibility else 0) | (16 if fallback else 0) | (32 if global_fallback else 0) | \
+        (64 if robust_fallback else 0)
+    check(lib().pmf_step_radiation(s.handle, float(dt), float(ch), float(decay), float(drag_impulse),
+                                   ctypes.c_void_p(correction.ptr), flags, _integ(integrator)))
+    return s
+
+
 def set_chunk(s: MHDState, zc: int):
     """Planes per z chunk of the fused kernel (0 = built-in choice); results do not depend on it."""
     check(lib().pmf_set_chunk(s.handle, int(zc)))

@@ -71,6 +71,12 @@ def lib():
             "pmf_surrogate_face_flux": [P, i64, i64, i32, d, P],
             "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
             "pmf_set_chunk": [P, i32],
+            "pmf_step_drag": [P, d, d, d, d, pd, i32],
+            "pmf_buffer_alloc": [i64, pp],
+            "pmf_buffer_free": [P],
+            "pmf_buffer_copy": [P, P, i64, i32],
+            "pmf_step_radiation": [P, d, d, d, d, P, i32, i32],
+            "pmf_step_sources_into": [P, P, P, P, P, d, d, d, d, pd, i32, i32],
             "pmf_autotune": [P, i32, ctypes.POINTER(ctypes.c_int32), pd],
             "pmf_ensemble_create": [ctypes.POINTER(pmf_params), i32, P, ctypes.POINTER(ctypes.c_void_p)],
             "pmf_ensemble_destroy": [P],

@@ -7,6 +7,9 @@
 * CT: the reference's fork-port cross-check (test/inherited/test_ct.jl:209-241) and its
   divergence / faces-from-cells invariants (test_ct.jl:243-334).
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -158,9 +161,10 @@ def alfven_L1(N, recon="plm", amp=0.1, p0=0.1, tfinal=0.5, cfl=0.4, glm_cr=O.GLM
     return L1, n, abs(tot1["mass"] - tot0["mass"]) / tot0["mass"], abs(tot1["energy"] - tot0["energy"]) / tot0["energy"]
 
 
-# validation-cpu.log:8-19 (Float32 @fastmath KA CPU reference, 3 significant figures)
-RECORDED = {"plm": ((6.919e-4, 48), (1.577e-4, 95), (3.586e-5, 190)),
-            "ppm": ((5.495e-4, 48), (1.519e-4, 95), (3.952e-5, 190))}
+# validation-cpu.log:8-19 (Float32 @fastmath KA CPU reference, 3 significant figures),
+# transcribed in tests/golden/known_answers.json
+KNOWN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "known_answers.json")))
+RECORDED = {r: tuple((e["L1"], e["steps"]) for e in KNOWN["alfven_circular"][r]) for r in ("plm", "ppm")}
 
 
 @pytest.mark.parametrize("recon", ["plm", "ppm"])
@@ -191,9 +195,10 @@ def test_alfven_float64_matches_recorded_tighter():
 
 def test_low_beta_alfven_matches_recorded():
     """runtests.jl:27-35; validation-cpu.log:5: L1 1.512e-4, 89 steps, dE 8.6e-8."""
-    L1, n, _, denergy = alfven_L1(64, amp=0.1, p0=1e-3, glm_cr=1.0)
-    assert n == 89
-    assert abs(L1 - 1.512e-4) / 1.512e-4 < 1e-2
+    K = KNOWN["alfven_low_beta"]
+    L1, n, _, denergy = alfven_L1(K["N"], amp=0.1, p0=1e-3, glm_cr=1.0)
+    assert n == K["steps"]
+    assert abs(L1 - K["L1"]) / K["L1"] < 1e-2
     assert denergy < 2e-4
 
 
@@ -213,8 +218,9 @@ def test_brio_wu_matches_recorded():
     r4, dm4, _ = brio_wu_density(512)
     r4d = 0.5 * (r4[0::2] + r4[1::2])
     L1 = np.mean(np.abs(r2 - r4d))
-    assert abs(L1 - 2.057e-3) / 2.057e-3 < 2e-2
-    assert abs(r2.min() - 0.117) < 1.5e-3
+    K = KNOWN["brio_wu"]
+    assert abs(L1 - K["L1_256_vs_512"]) / K["L1_256_vs_512"] < 2e-2
+    assert abs(r2.min() - K["rho_min"]) < 1.5e-3
     assert r2[0] > 0.9 and r2[-1] < 0.2 and r2.max() < 1.02
     assert dm4 < 1e-4
 
@@ -232,8 +238,8 @@ def test_orszag_tang_matches_recorded():
     h = s.U.astype(np.float64)
     brms = np.sqrt(np.mean(h[5] ** 2 + h[6] ** 2 + h[7] ** 2))
     norm = s.max_divb() * float(s.dx) / brms
-    assert n == 333
-    assert abs(norm - 2.79) < 0.05
+    assert n == KNOWN["orszag_tang"]["steps"]
+    assert abs(norm - KNOWN["orszag_tang"]["max_divb_dx_over_brms"]) < 0.05
     assert abs(t1["mass"] - t0["mass"]) / t0["mass"] < 1e-4
     assert h[0].min() > 0
 
@@ -254,9 +260,10 @@ def test_field_loop_matches_recorded():
     l1 = (np.abs(h1[5] - h0[5]).sum() + np.abs(h1[6] - h0[6]).sum()) / bnorm
     emag1 = (h1[5] ** 2).sum() + (h1[6] ** 2).sum()
     divn = s.max_divb() * float(s.dx) / np.sqrt(emag1 / h1[5].size)
-    assert n == 734
-    assert abs(l1 - 0.1874) < 2e-3
-    assert abs(emag1 / emag0 - 0.895) < 2e-3
+    K = KNOWN["field_loop"]
+    assert n == K["steps"]
+    assert abs(l1 - K["L1"]) < 2e-3
+    assert abs(emag1 / emag0 - K["emag_retention"]) < 2e-3
     assert divn < 0.5
 
 

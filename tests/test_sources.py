@@ -209,3 +209,194 @@ def test_drag_propagates_the_face_velocity_to_the_half_step():
     b.step_sources(dt, 1.0, 1.0, drag_impulse=q)
     ratio = np.linalg.norm(b.U[0] - U[0]) / np.linalg.norm(a.U[0] - U[0])
     assert ratio == pytest.approx(np.exp(-q / 2), rel=0.03)
+
+
+# ---------------------------------------------------------------------------
+# GPU kernels against the oracle (float64: 1e-12; float32: 1e-5)
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-30)
+    return float((np.abs(a - b) / scale).max())
+
+
+def _corr(n, U, seed=0, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    c = np.zeros((9, n ** 3))
+    c[0] = U[8]
+    c[1:4] = rng.normal(0, 0.02, (3, n ** 3))
+    c[4] = rng.normal(0, 1e-3, n ** 3)
+    c[5:9] = rng.normal(0, 1e-7, (4, n ** 3))
+    return np.ascontiguousarray(c.astype(dtype))
+
+
+def _pair(n, U, dtype, recon, riemann, bcs=("periodic",) * 3, solver="glm"):
+    import pmfmhd as M
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=dtype, recon=recon, riemann=riemann, bcs=bcs, mhd_solver=solver)
+    o = O.OracleState((n, n, n), 1.0 / n, dtype=dtype, recon=recon, riemann=riemann, bcs=bcs, mhd_solver=solver)
+    s.set_conserved(U)
+    o.set_conserved(U)
+    return s, o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integ", ["ref", "cube"])
+@pytest.mark.parametrize("recon,riemann", [("plm", "hlld"), ("ppm", "hll"), ("plm_minmod", "llf")])
+def test_drag_step_matches_oracle_float64(integ, recon, riemann):
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=7)
+    s, o = _pair(n, U, np.float64, recon, riemann)
+    dt, ch = o.compute_dt(0.3)
+    for _ in range(3):
+        M.step_drag(s, dt, ch, 0.95, 0.3, integrator=integ)
+        o.step_sources(dt, ch, 0.95, drag_impulse=0.3)
+    assert _rel(s.conserved(), o.U) < 1e-12
+
+
+@pytest.mark.gpu
+def test_ref_drag_with_a_moving_target_and_boundaries_matches_oracle():
+    import pmfmhd as M
+    n = 12
+    U = smooth_state(n, seed=8)
+    s, o = _pair(n, U, np.float64, "plm", "hlld", bcs=("outflow", "reflecting", "periodic"))
+    dt, ch = o.compute_dt(0.3)
+    v0 = (0.1, -0.2, 0.05)
+    for _ in range(2):
+        M.step_drag(s, dt, ch, 0.97, 0.8, target_velocity=v0, integrator="ref")
+        o.step_sources(dt, ch, 0.97, drag_impulse=0.8, target_velocity=v0)
+    assert _rel(s.conserved(), o.U) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integ", ["ref", "cube"])
+def test_radiation_step_with_tracking_matches_oracle_float64(integ):
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=9)
+    s, o = _pair(n, U, np.float64, "plm", "hlld")
+    c = _corr(n, U, seed=1)
+    buf = M.correction_buffer(s, c)
+    dt, ch = o.compute_dt(0.3)
+    for _ in range(2):
+        M.step_radiation(s, dt, ch, 0.9, 0.4, buf, integrator=integ)
+        o.step_sources(dt, ch, 0.9, drag_impulse=0.4, correction=c, track_density=True, track_magnetic=True,
+                       track_energy=True)
+        c[0] = o.U[8]  # the driver reloads psi into slot 0 before each step
+        M.load_psi(s, buf)
+    assert _rel(s.conserved(), o.U) < 1e-12
+    got = buf.download(np.float64, (9, n ** 3))
+    assert _rel(got[1:4], c[1:4]) == 0.0
+    assert _rel(got[4], c[4]) < 1e-12
+    # residual slots hold rounding errors: compare in absolute terms against the field scale
+    for v in (5, 6, 7, 8):
+        assert np.abs(got[v] - c[v]).max() < 1e-13 * max(1.0, np.abs(o.U[v if v < 8 else 4]).max())
+
+
+@pytest.mark.gpu
+def test_ct_radiation_step_matches_oracle_float64():
+    import pmfmhd as M
+    from test_oracle import ct_test_hydro, ct_test_potential
+    n = 12
+    U = ct_test_hydro(n, G)
+    s, o = _pair(n, U, np.float64, "plm", "hll", solver="ct")
+    Ax, Ay, Az = ct_test_potential(n)
+    M.ct_faces_from_potential(s, Ax, Ay, Az)
+    o.ct_faces_from_potential(Ax, Ay, Az)
+    c = _corr(n, o.U, seed=2)
+    buf = M.correction_buffer(s, c)
+    dt, _ = o.compute_dt(0.3)
+    for _ in range(2):
+        M.step_radiation(s, dt, 0.0, 1.0, 0.2, buf, integrator="ref")
+        o.step_sources(dt, 0.0, 1.0, drag_impulse=0.2, correction=c, track_density=True, track_magnetic=True,
+                       track_energy=True)
+    assert _rel(s.conserved(), o.U) < 1e-12
+    assert _rel(s.ct_slot(0), o.ct[0]) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["check", "fallback", "global", "robust"])
+def test_cube_radiation_checks_and_fallbacks_match_oracle(mode):
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=11)
+    cbad = 5 * n * n + 3 * n + 9
+    U[4, cbad] = 0.5 * (U[1, cbad] ** 2 + U[2, cbad] ** 2 + U[3, cbad] ** 2) / U[0, cbad] + \
+        0.5 * (U[5, cbad] ** 2 + U[6, cbad] ** 2 + U[7, cbad] ** 2) - 0.3
+    s, o = _pair(n, U, np.float64, "plm", "hlld")
+    c = _corr(n, U, seed=3)
+    if mode == "fallback":
+        c[4] = (np.random.default_rng(4).random(n ** 3) < 0.1).astype(np.float64)
+    buf = M.correction_buffer(s, c)
+    dt, ch = 2e-3, 2.0
+    kw = dict(check_admissibility=mode == "check", fallback=mode == "fallback", global_fallback=mode == "global",
+              robust_fallback=mode == "robust", track_density=mode != "fallback")
+    M.step_radiation(s, dt, ch, 0.9, 0.1, buf, integrator="cube", **kw)
+    o.step_sources(dt, ch, 0.9, drag_impulse=0.1, correction=c, track_density=mode != "fallback",
+                   track_magnetic=True, track_energy=True,
+                   check={"check": 1, "fallback": 2, "global": 3, "robust": 0}[mode],
+                   recon="plm_minmod" if mode == "robust" else None,
+                   riemann={"global": "llf", "robust": "hll"}.get(mode))
+    got = s.conserved()
+    assert _rel(got, o.U) < 1e-12
+    if mode == "check":
+        assert got[0, cbad] < 0 and (got[0] < 0).sum() == (o.U[0] < 0).sum()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integ", ["ref", "cube"])
+def test_radiation_float32_against_float32_oracle(integ):
+    import pmfmhd as M
+    from pmfmhd import problems as P
+    n = 32
+    U = P.config_state("c1_32", seed=4).astype(np.float32)
+    s, o = _pair(n, U, np.float32, "plm_minmod", "hlld")
+    c = _corr(n, U, seed=5, dtype=np.float32)
+    buf = M.correction_buffer(s, c)
+    dt, ch = o.compute_dt(0.2)
+    M.step_radiation(s, dt, ch, 0.99, 0.05, buf, integrator=integ)
+    o.step_sources(dt, ch, 0.99, drag_impulse=0.05, correction=c, track_density=True, track_magnetic=True,
+                   track_energy=True)
+    got = s.conserved().astype(np.float64)
+    ref = o.U.astype(np.float64)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_drag_ref_and_cube_agree_float32():
+    """The reference's own check (test_cube_cpu.jl:92-101): the drag step is the same on
+    both integrators."""
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=12).astype(np.float32)
+    a, _ = _pair(n, U, np.float32, "ppm", "hlld")
+    b, _ = _pair(n, U, np.float32, "ppm", "hlld")
+    dt, ch = M.compute_dt(a, 0.08)
+    for _ in range(3):
+        M.step_drag(a, dt, ch, 0.99, 0.05, integrator="ref")
+        M.step_drag(b, dt, ch, 0.99, 0.05, integrator="cube")
+    ua, ub = a.conserved().astype(np.float64), b.conserved().astype(np.float64)
+    assert np.abs(ua - ub).max() / np.abs(ua).max() < 2e-6
+    mass0 = U[0].astype(np.float64).sum()
+    assert abs(ub[0].sum() / mass0 - 1) < 3e-6
+
+
+@pytest.mark.gpu
+def test_coupled_step_error_texts():
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=1)
+    s, _ = _pair(n, U, np.float32, "plm", "hlld")
+    buf = M.correction_buffer(s)
+    with pytest.raises(M.MHDError, match="specialized for Compton drag in the CMB frame"):
+        M.step_drag(s, 1e-3, 1.0, 1.0, 0.1, target_velocity=(0.1, 0, 0), integrator="cube")
+    with pytest.raises(M.MHDError, match="options of the :cube integrator"):
+        M.step_radiation(s, 1e-3, 1.0, 1.0, 0.1, buf, check_admissibility=True, integrator="ref")
+    with pytest.raises(M.MHDError, match="density tracking writes"):
+        M.step_radiation(s, 1e-3, 1.0, 1.0, 0.1, buf, fallback=True, integrator="cube")
+    with pytest.raises(M.MHDError, match="needs the GLM cleaning speed"):
+        M.step_drag(s, 1e-3, 0.0, 1.0, 0.1, integrator="ref")
+    c, _ = _pair(n, U, np.float32, "plm", "hll", solver="ct")
+    with pytest.raises(M.MHDError, match="step_cube_drag!: CT is implemented on the :ref integrator only"):
+        M.step_drag(c, 1e-3, 0.0, 1.0, 0.1, integrator="cube")
