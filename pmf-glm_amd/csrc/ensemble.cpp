@@ -153,9 +153,12 @@ int pmf_ensemble_create(const pmf_params* base, int32_t ncubes, const int32_t* e
     if (!(base->gamma > 1.0)) throw Error("gamma must exceed 1");
     if (base->riemann < 0 || base->riemann > 2) throw Error("unknown riemann solver code");
     if (base->recon < 0 || base->recon > 2) throw Error("unknown reconstruction code");
-    for (int c = 0; c < ncubes; ++c)
+    for (int c = 0; c < ncubes; ++c) {
       if (edge[c] < 16 || edge[c] % kFusedTX)
         throw Error("ensemble cube edges must be multiples of 16 (got " + std::to_string(edge[c]) + ")");
+      if (edge[c] > 1008)  // each field below 4 GiB: the step addresses cubes with 32-bit offsets
+        throw Error("ensemble cube edges must be at most 1008 (got " + std::to_string(edge[c]) + ")");
+    }
     auto* e = new pmf_ensemble();
     try {
       e->p = *base;

@@ -20,6 +20,8 @@
 // once, and the global state is read once and written once per step. Workgroups are
 // remapped so that consecutive tiles land on the same XCD (and its L2), which then
 // serves the halo re-reads.
+#include <type_traits>
+
 #include "launch.hpp"
 #include "surrogate.hpp"
 
@@ -41,11 +43,22 @@ __device__ __forceinline__ int wrapi(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-template <class T>
-__device__ __forceinline__ void load_cons(const Fields<const T>& U, int64_t idx, int64_t nc, T* c) {
-  (void)nc;
+// Address of cell idx of one field. NARROW: the byte offset fits 32 bits (fields below
+// 4 GiB), so every access is the field's scalar base plus one 32-bit vector offset shared
+// by the nine fields (global_load/store with saddr) instead of a 64-bit address each.
+template <bool NARROW, class T> __device__ __forceinline__ T* cell_at(T* base, int64_t idx) {
+  if constexpr (NARROW) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (uint32_t)idx * (uint32_t)sizeof(T));
+  } else {
+    return base + idx;
+  }
+}
+
+template <bool NARROW, class T>
+__device__ __forceinline__ void load_cons(const Fields<const T>& U, int64_t idx, T* c) {
 #pragma unroll
-  for (int v = 0; v < 9; ++v) c[v] = U.v[v][idx];
+  for (int v = 0; v < 9; ++v) c[v] = *cell_at<NARROW>(U.v[v], idx);
 }
 
 // Ring cell h (0..NRING-1) at distance r from the tile: bottom row, top row, left
@@ -108,7 +121,7 @@ struct FusedLds {
 // learned flux (wave-level MFMA) or by HLLD; sW/sB hold the network's LDS image
 // (surrogate.hpp) and smooth_count counts the faces that used it (each face once, as
 // the lower face of its cell).
-template <class T, int RECON, int RS, bool SURR, bool SRC = false>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
@@ -142,11 +155,11 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   {
     T c[9], q[9];
     if (has_col) {
-      load_cons(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, g.nc, c);
+      load_cons<NARROW>(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, c);
       cons2prim(c, q, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) S.sP[(z0 - 2) & 1][v * FNP + me] = q[v];
-      load_cons(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, g.nc, c);
+      load_cons<NARROW>(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, c);
       cons2prim(c, q, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) S.sP[(z0 - 1) & 1][v * FNP + me] = q[v];
@@ -166,7 +179,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     if (m - 1 >= z0 && stores) {
       T o[9];
       const int64_t idx = (int64_t)(m - 1) * g.plane + col;
-      load_cons(U, idx, g.nc, o);
+      load_cons<NARROW>(U, idx, o);
       if constexpr (SRC) {
         // tracked increments, drag propagator, admissibility mark (integrator_cube.jl:423-441)
         T du[9], U0[9];
@@ -182,7 +195,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       }
       o[8] *= P.decay;
 #pragma unroll
-      for (int v = 0; v < 9; ++v) O.v[v][idx] = o[v];
+      for (int v = 0; v < 9; ++v) *cell_at<NARROW>(O.v[v], idx) = o[v];
       smax = nanmax(smax, cell_wavespeed(o, P));
     }
     if (!first) {
@@ -202,9 +215,9 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     T* __restrict__ nxt = S.sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
     // (1) plane m+1 of the own column and of the extra cell (in flight during the barrier)
     T Pn[9], Xc[9];
-    if (has_col) load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, g.nc, Pn);
+    if (has_col) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, Pn);
     const bool extra_now = has_extra && m + 1 < z0 + zc;
-    if (extra_now) load_cons(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, g.nc, Xc);
+    if (extra_now) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, Xc);
     __syncthreads();
     if (extra_now) {
       T q[9];
@@ -438,7 +451,7 @@ constexpr bool fused_three_waves() {
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
 // global memory (SURR only) and nsmooth the count of network-solved faces.
-template <class T, int RECON, int RS, bool SURR, bool SRC>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
 __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                                 const StepParams<T>& P, double* __restrict__ red,
                                                 const unsigned char* __restrict__ simg, float thr,
@@ -460,7 +473,7 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
-  const T smax = fused_item<T, RECON, RS, SURR, SRC>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src);
+  const T smax = fused_item<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src);
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
   block_store_max(smax, sMax, red + blockIdx.x);
   if constexpr (SURR) {
@@ -470,32 +483,43 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   }
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
                                                          StepParams<T> P, double* __restrict__ red,
                                                          const unsigned char* __restrict__ simg, float thr,
                                                          unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
-  step_fused_body<T, RECON, RS, SURR, SRC>(U, O, g, P, red, simg, thr, nsmooth, src);
+  step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused3(
     Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
     const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
-  step_fused_body<T, RECON, RS, SURR, SRC>(U, O, g, P, red, simg, thr, nsmooth, src);
+  step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC = false>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
 void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                           const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
                           unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{}) {
   if constexpr (fused_three_waves<T, RECON, RS, SURR, SRC>())
-    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg,
-                       thr, nsmooth, src);
+    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
+                       simg, thr, nsmooth, src);
   else
-    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red, simg,
-                       thr, nsmooth, src);
+    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
+                       simg, thr, nsmooth, src);
 }
+
+// The plain float step with 32-bit offsets (double steps never take this path).
+template <class T, int RECON, int RS>
+void launch_plain_narrow(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
+                         const StepParams<T>& P, double* red) {
+  if constexpr (sizeof(T) == 4)
+    launch_fused_variant<T, RECON, RS, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr);
+}
+
+// Fields of nc cells addressable with 32-bit byte offsets.
+template <class T> inline bool narrow_fields(int64_t nc) { return (uint64_t)nc * sizeof(T) < (1ull << 32); }
 
 __global__ __launch_bounds__(256) void k_fold_max_flat(const double* __restrict__ red, int nb, double* __restrict__ out) {
   __shared__ double sh[4];
@@ -554,8 +578,10 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
+  const bool narrow = sizeof(T) == 4 && narrow_fields<T>(G.nc);
 #define PMF_FUSED(R, S)                                                                                          \
   if (src) launch_fused_variant<T, R, S, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src);    \
+  else if (narrow) launch_plain_narrow<T, R, S>(grd, st, U, O, g, P, red);                                        \
   else launch_fused_variant<T, R, S, false, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
@@ -705,7 +731,8 @@ __device__ __forceinline__ void ens_step_body(const EnsCube* __restrict__ cubes,
     for (int v = 0; v < 9; ++v) U.v[v] = in.v[v];
     const StepParams<float> P = params[c];
     const float smax =
-        fused_item<float, RECON, RS, false>(U, out, cb.g, P, w, lds, nullptr, nullptr, 0.f, none, SrcParams<float>{});
+        fused_item<float, RECON, RS, false, false, true>(U, out, cb.g, P, w, lds, nullptr, nullptr, 0.f, none,
+                                                         SrcParams<float>{});
     block_store_max(smax, sMax, red + i);
     __syncthreads();  // sItem and sMax are reused by the next item
   }
