@@ -116,6 +116,25 @@ def error_report(M, P, n, steps):
             "rel_l2_fp32_vs_fp64": rel_l2(got, ref)}
 
 
+def fp64_oracle_check(M, P, n, steps):
+    """The float64 GPU step against the float64 CPU oracle at full size: the GPU run's
+    (dt, ch) sequence replayed by the oracle (OpenMP over the host cores)."""
+    import numpy as np
+    import oracle as O
+    U = P.config_state("c2_256", seed=1, n=n)
+    gpu, seq = fp64_sequence(M, U, n, steps)
+    s = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
+    s.set_conserved(U)
+    t0 = time.perf_counter()
+    for dt, ch in seq:
+        s.step(dt, ch=ch)
+    el = time.perf_counter() - t0
+    diff = np.abs(gpu - s.U).max(axis=1) / np.maximum(np.abs(s.U).max(axis=1), 1e-300)
+    return {"workload": f"plain GLM-HLLD float64 {n}^3, seed 1, {steps} steps, oracle replaying the GPU dt sequence",
+            "max_rel_diff_per_variable": [float(x) for x in diff], "oracle_seconds": el,
+            "oracle_cores": O.num_threads()}
+
+
 def surrogate_leg(M, P, n, steps, weights, thr):
     """The surrogate-assisted step against the plain step on the 32^3-recipe inputs at n^3
     (rms |B| 0.3, |m| <= 8, v = 0): throughput of both, and each one's whole-run error
@@ -245,6 +264,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in z-chunk depth")
     ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
+    ap.add_argument("--fp64-oracle-steps", type=int, default=0,
+                    help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~4 s per step)")
     ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
@@ -338,7 +359,10 @@ def main():
     if not args.no_ensemble:
         log("bench: ensemble leg")
         ens = ensemble_leg(P)
-    c3 = ct = None
+    c3 = ct = orc = None
+    if args.fp64_oracle_steps > 0 and integ == "cube":
+        log(f"bench: float64 GPU vs CPU oracle ({args.fp64_oracle_steps} steps)")
+        orc = fp64_oracle_check(M, P, n, args.fp64_oracle_steps)
     if args.c3_steps > 0:
         log(f"bench: 512^3 leg ({args.c3_steps} steps)")
         c3 = c3_leg(M, P, args.c3_steps)
@@ -373,6 +397,7 @@ def main():
         "ensemble": ens,
         "c3_512": c3,
         "ct": ct,
+        "fp64_oracle_check": orc,
     }
     print(json.dumps(out), flush=True)
 

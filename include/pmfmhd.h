@@ -199,6 +199,19 @@ int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, vo
                           double ch, double decay, double drag_impulse, const double* target_velocity,
                           int32_t flags, int32_t integrator);
 
+/* ---- Passive species (passive_transport.jl) --------------------------------------
+ * conserved_species_to_fraction(s)!: species densities to bounded mass fractions in
+ * place, using rho (NULL: the state's density); with two species the second bounds the
+ * first (q1 <= maximum_total - q2). species2 may be NULL. */
+int pmf_species_to_fractions(pmf_state* s, void* species1, void* species2, const void* rho, double maximum_total);
+/* advect_species_fraction(s)!: backtrace the fractions with the state's post-step cell
+ * velocity (cubic periodic grid, trilinear), restore mass densities with the new gas
+ * density, via the caller's scratch fields. species2 / scratch2 may be NULL. */
+int pmf_advect_species(pmf_state* s, void* species1, void* species2, void* scratch1, void* scratch2, double dt,
+                       double maximum_total);
+/* advect_passive_scalar!: the same backtrace of an unconstrained scalar (no bounds, no density). */
+int pmf_advect_passive_scalar(pmf_state* s, void* scalar, void* scratch, double dt);
+
 /* Device buffers for callers without a GPU array library (e.g. the radiation
  * correction buffer): zero-filled allocation, free, and copies (kind 1 host to device,
  * 2 device to host, 3 device to device; the copies wait for the device). */

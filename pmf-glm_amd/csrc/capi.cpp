@@ -747,6 +747,60 @@ int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, vo
   });
 }
 
+int pmf_species_to_fractions(pmf_state* s, void* species1, void* species2, const void* rho, double maximum_total) {
+  return guard([&] {
+    check_state(s);
+    if (!species1) throw Error("null species field");
+    if (!rho && !s->U) throw Error("pass the gas density (this state is a workspace without fields)");
+    const void* r = rho ? rho : s->U;  // the density is the first field
+    if (s->p.precision == PMF_F64)
+      launch_species_to_fractions<double>((double*)species1, (double*)species2, (const double*)r, s->nc,
+                                          maximum_total, s->st);
+    else
+      launch_species_to_fractions<float>((float*)species1, (float*)species2, (const float*)r, s->nc,
+                                         float(maximum_total), s->st);
+    hip_check(hipGetLastError(), "species launch");
+    sync(s);
+  });
+}
+
+namespace {
+void backtrace_call(pmf_state* s, void* a1, void* a2, void* t1, void* t2, double dt, double maxt, int mode,
+                    const char* what) {
+  check_fields(s);
+  if (!a1 || !t1 || (mode == 1 && (!a2 || !t2))) throw Error("null field or scratch");
+  if (!(s->p.nx == s->p.ny && s->p.ny == s->p.nz))
+    throw Error(std::string(what) + " remap currently requires a cubic grid");
+  const int n = s->p.nx;
+  const size_t bytes = size_t(s->nc) * s->esz;
+  if (s->p.precision == PMF_F64)
+    launch_backtrace<double>((double*)t1, (double*)t2, (const double*)a1, (const double*)a2,
+                             cfields_at<double>(s->U, s->nc), n, dt / s->p.dx, maxt, mode, s->st);
+  else
+    launch_backtrace<float>((float*)t1, (float*)t2, (const float*)a1, (const float*)a2,
+                            cfields_at<float>(s->U, s->nc), n, float(dt) / float(s->p.dx), float(maxt), mode, s->st);
+  hip_check(hipGetLastError(), "backtrace launch");
+  hip_check(hipMemcpyAsync(a1, t1, bytes, hipMemcpyDeviceToDevice, s->st), "copy");
+  if (mode == 1) hip_check(hipMemcpyAsync(a2, t2, bytes, hipMemcpyDeviceToDevice, s->st), "copy");
+  sync(s);
+}
+}  // namespace
+
+int pmf_advect_species(pmf_state* s, void* species1, void* species2, void* scratch1, void* scratch2, double dt,
+                       double maximum_total) {
+  return guard([&] {
+    check_state(s);
+    backtrace_call(s, species1, species2, scratch1, scratch2, dt, maximum_total, species2 ? 1 : 0, "species");
+  });
+}
+
+int pmf_advect_passive_scalar(pmf_state* s, void* scalar, void* scratch, double dt) {
+  return guard([&] {
+    check_state(s);
+    backtrace_call(s, scalar, nullptr, scratch, nullptr, dt, 0.0, 2, "scalar");
+  });
+}
+
 int pmf_buffer_alloc(int64_t bytes, void** out) {
   return guard([&] {
     if (!out || bytes <= 0) throw Error("bad buffer request");

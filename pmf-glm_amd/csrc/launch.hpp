@@ -68,6 +68,14 @@ void launch_ens_params(const EnsCube* cubes, int ncubes, const double* red, cons
 void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q, const StepParams<float>* params,
                      double* red, unsigned* ctr, int nwg, int parity, int recon, int rsolve, hipStream_t st);
 
+// Passive species (passive_transport.jl): fractions from conserved densities in place
+// (s2 may be null), and the backtrace into o1 (o2) of fractions (mode 0 / 1) or of an
+// unconstrained scalar (mode 2) with the post-step velocity of U on an n^3 periodic grid.
+template <class T> void launch_species_to_fractions(T* s1, T* s2, const T* rho, int64_t nc, T maxt, hipStream_t st);
+template <class T>
+void launch_backtrace(T* o1, T* o2, const T* a1, const T* a2, Fields<const T> U, int n, T dtdx, T maxt, int mode,
+                      hipStream_t st);
+
 template <class T>
 void launch_wavespeed(Fields<const T> U, const Grid& G, const StepParams<T>& P, double* red, double* out,
                       hipStream_t st);

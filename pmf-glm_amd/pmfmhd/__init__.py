@@ -283,6 +283,28 @@ def load_psi(s: MHDState, corr: DeviceBuffer):
     check(lib().pmf_buffer_copy(ctypes.c_void_p(corr.ptr), ctypes.c_void_p(p), s.ncells * s.dtype.itemsize, 3))
 
 
+def _vp(b):
+    return None if b is None else ctypes.c_void_p(b.ptr)
+
+
+def species_to_fractions(s: MHDState, species1: DeviceBuffer, species2: DeviceBuffer = None, maximum_total=1.0):
+    """conserved_species_to_fraction(s)! (passive_transport.jl:5-25, 198-229) with the
+    state's density."""
+    check(lib().pmf_species_to_fractions(s.handle, _vp(species1), _vp(species2), None, float(maximum_total)))
+
+
+def advect_species(s: MHDState, species1: DeviceBuffer, scratch1: DeviceBuffer, dt, species2: DeviceBuffer = None,
+                   scratch2: DeviceBuffer = None, maximum_total=1.0):
+    """advect_species_fraction(s)! (passive_transport.jl:231-265)."""
+    check(lib().pmf_advect_species(s.handle, _vp(species1), _vp(species2), _vp(scratch1), _vp(scratch2), float(dt),
+                                   float(maximum_total)))
+
+
+def advect_passive_scalar(s: MHDState, scalar: DeviceBuffer, scratch: DeviceBuffer, dt):
+    """advect_passive_scalar! (passive_transport.jl:267-289)."""
+    check(lib().pmf_advect_passive_scalar(s.handle, _vp(scalar), _vp(scratch), float(dt)))
+
+
 def step_drag(s: MHDState, dt, ch, decay, drag_impulse, target_velocity=(0.0, 0.0, 0.0), integrator="ref"):
     """step_ref_drag! / step_cube_drag! (integrator_ref.jl:222-236, integrator_cube.jl:478-503)."""
     v0 = (ctypes.c_double * 3)(*[float(v) for v in target_velocity])

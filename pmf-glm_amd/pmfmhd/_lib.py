@@ -73,6 +73,9 @@ def lib():
             "pmf_set_chunk": [P, i32],
             "pmf_step_drag": [P, d, d, d, d, pd, i32],
             "pmf_buffer_alloc": [i64, pp],
+            "pmf_species_to_fractions": [P, P, P, P, d],
+            "pmf_advect_species": [P, P, P, P, P, d, d],
+            "pmf_advect_passive_scalar": [P, P, P, d],
             "pmf_buffer_free": [P],
             "pmf_buffer_copy": [P, P, i64, i32],
             "pmf_step_radiation": [P, d, d, d, d, P, i32, i32],
