@@ -807,6 +807,8 @@ int pmf_buffer_alloc(int64_t bytes, void** out) {
     *out = nullptr;
     hip_check(hipMalloc(out, size_t(bytes)), "hipMalloc");
     hip_check(hipMemset(*out, 0, size_t(bytes)), "hipMemset");
+    // the fill runs on the null stream, which the library's non-blocking streams do not wait for
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   });
 }
 
@@ -823,6 +825,9 @@ int pmf_buffer_copy(void* dst, const void* src, int64_t bytes, int32_t kind) {
     hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     hip_check(hipMemcpy(dst, src, size_t(bytes), k), "hipMemcpy");
+    // hipMemcpy from pageable host memory (and device to device) may return before the data
+    // has landed; work queued next on a non-blocking stream is not ordered after it
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   });
 }
 

@@ -110,8 +110,13 @@ void build_items(pmf_ensemble* e) {
     hip_check(hipMalloc(&e->ditems, sizeof(int2) * e->item_cap), "hipMalloc");
     hip_check(hipMalloc(&e->red, sizeof(double) * e->item_cap), "hipMalloc");
   }
-  hip_check(hipMemcpy(e->ditems, e->items.data(), sizeof(int2) * e->items.size(), hipMemcpyHostToDevice), "copy");
-  hip_check(hipMemcpy(e->dcubes, e->cubes.data(), sizeof(EnsCube) * e->ncubes, hipMemcpyHostToDevice), "copy");
+  // on the ensemble's stream, so the copies are ordered after launches already queued there
+  // and before the next ones; the wait keeps the host vectors alive until they are read
+  hip_check(hipMemcpyAsync(e->ditems, e->items.data(), sizeof(int2) * e->items.size(), hipMemcpyHostToDevice, e->st),
+            "copy");
+  hip_check(hipMemcpyAsync(e->dcubes, e->cubes.data(), sizeof(EnsCube) * e->ncubes, hipMemcpyHostToDevice, e->st),
+            "copy");
+  hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
   drop_graph(e);
 }
 
@@ -187,6 +192,8 @@ int pmf_ensemble_create(const pmf_params* base, int32_t ncubes, const int32_t* e
       hip_check(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking), "hipStreamCreate");
       hip_check(hipEventCreate(&e->e0), "hipEventCreate");
       hip_check(hipEventCreate(&e->e1), "hipEventCreate");
+      // the fills above ran on the null stream, which the ensemble's stream does not wait for
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       build_items(e);
     } catch (...) {
       pmf_ensemble_destroy(e);
@@ -221,8 +228,9 @@ int pmf_ensemble_upload(pmf_ensemble* e, int32_t cube, int32_t var, const void* 
     const int64_t nc = int64_t(e->edge[cube]) * e->edge[cube] * e->edge[cube];
     if (n != nc) throw Error("upload size mismatch");
     if (!host) throw Error("null argument");
+    hip_check(hipMemcpyAsync(current(e, cube).v[var], host, sizeof(float) * nc, hipMemcpyHostToDevice, e->st),
+              "copy");
     hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
-    hip_check(hipMemcpy(current(e, cube).v[var], host, sizeof(float) * nc, hipMemcpyHostToDevice), "copy");
   });
 }
 
@@ -233,8 +241,9 @@ int pmf_ensemble_download(pmf_ensemble* e, int32_t cube, int32_t var, void* host
     const int64_t nc = int64_t(e->edge[cube]) * e->edge[cube] * e->edge[cube];
     if (n != nc) throw Error("download size mismatch");
     if (!host) throw Error("null argument");
+    hip_check(hipMemcpyAsync(host, current(e, cube).v[var], sizeof(float) * nc, hipMemcpyDeviceToHost, e->st),
+              "copy");
     hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
-    hip_check(hipMemcpy(host, current(e, cube).v[var], sizeof(float) * nc, hipMemcpyDeviceToHost), "copy");
   });
 }
 
@@ -283,8 +292,8 @@ int pmf_ensemble_clock(pmf_ensemble* e, double* t, double* dt, double* smax) {
   return guard([&] {
     check_ens(e);
     std::vector<double> h(3 * e->ncubes);
+    hip_check(hipMemcpyAsync(h.data(), e->clock, sizeof(double) * h.size(), hipMemcpyDeviceToHost, e->st), "copy");
     hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
-    hip_check(hipMemcpy(h.data(), e->clock, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "copy");
     for (int c = 0; c < e->ncubes; ++c) {
       if (t) t[c] = h[3 * c];
       if (dt) dt[c] = h[3 * c + 1];
@@ -328,8 +337,9 @@ int pmf_ensemble_tune(pmf_ensemble* e, int32_t reps, double* ms_per_step) {
     if (reps < 1) throw Error("reps must be positive");
     drop_graph(e);
     std::vector<double> saved(3 * e->ncubes);
+    hip_check(hipMemcpyAsync(saved.data(), e->clock, sizeof(double) * saved.size(), hipMemcpyDeviceToHost, e->st),
+              "copy");
     hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
-    hip_check(hipMemcpy(saved.data(), e->clock, sizeof(double) * saved.size(), hipMemcpyDeviceToHost), "copy");
     auto measure = [&]() {
       build_items(e);
       initial_speeds(e);
@@ -370,7 +380,9 @@ int pmf_ensemble_tune(pmf_ensemble* e, int32_t reps, double* ms_per_step) {
     }
     e->nwg = best_nwg;
     build_items(e);
-    hip_check(hipMemcpy(e->clock, saved.data(), sizeof(double) * saved.size(), hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpyAsync(e->clock, saved.data(), sizeof(double) * saved.size(), hipMemcpyHostToDevice, e->st),
+              "copy");
+    hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
     if (ms_per_step) *ms_per_step = best;
   });
 }

@@ -423,6 +423,8 @@ int pmf_surrogate_create(pmf_state* s, uint64_t seed, pmf_surrogate** out) {
       hip_check(hipMemcpy(n->w, h.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice), "copy");
       hip_check(hipMemset(n->m, 0, sizeof(float) * NPARAM), "memset");
       hip_check(hipMemset(n->v, 0, sizeof(float) * NPARAM), "memset");
+      // null-stream work: the state's non-blocking stream does not wait for it
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
       pmf_surrogate_destroy(n);
       throw;
