@@ -37,6 +37,7 @@ constexpr int FNQ = FQW * (TY + 2);    // 252
 constexpr int FNFX = (TX + 1) * TY;    // x faces of a tile plane: 204
 constexpr int FNFY = TX * (TY + 1);    // y faces of a tile plane: 208
 constexpr int FTHREADS = 256;
+constexpr int FQCAP = NOWN + FNFX + FNFY;  // faces of one tile plane (z, x, y): the surrogate queue
 
 __device__ __forceinline__ int wrapi(int i, int n) {
   int r = i % n;
@@ -125,7 +126,8 @@ template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW =
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
-                                        float thr, unsigned& smooth_count, const SrcParams<T>& src) {
+                                        float thr, unsigned& smooth_count, const SrcParams<T>& src,
+                                        SurrQueue<FQCAP>* Qp = nullptr) {
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -218,6 +220,9 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     if (has_col) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, Pn);
     const bool extra_now = has_extra && m + 1 < z0 + zc;
     if (extra_now) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, Xc);
+    if constexpr (SURR) {
+      if (t == 0) Qp->n = 0u;  // the previous plane's queue was last read before its final barrier
+    }
     __syncthreads();
     if (extra_now) {
       T q[9];
@@ -344,45 +349,84 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
       for (int v = 0; v < 9; ++v) nxt[v * FNP + me] = Pn[v];
     }
-    if constexpr (SURR) {
-      // waves 0-2 hold exactly the owned columns, so this branch is wave-uniform
-      if (__builtin_amdgcn_readfirstlane(t) < NOWN) {
-        T Fz[9];
-        // counts each face once: the lower faces of the stored cells
-        if (!first) smooth_count += surr_face<2>(Lzp, Rz, Fz, P, thr, true, sW, sB) && stores && !last;
-        zface_update(Fz, Lzn, m, first);
-      }
-    }
     __syncthreads();
     // (3b) x and y faces of plane m, each solved once
     const bool xy = !(first || last);
-    if (xy) {
+    if constexpr (SURR) {
+      // Surrogate-assisted faces of plane m: (a) every face of the plane (z, x, y) is
+      // gated; rough faces take HLLD now and smooth ones queue their network inputs;
+      // (b) the waves evaluate the queue in dense batches of 32 (one MFMA pass each);
+      // (c) each smooth face adds its learned dissipation to its central flux. Face
+      // states were read in (a), so the fluxes may overwrite them after (b)'s barriers.
+      SurrQueue<FQCAP>& Q = *Qp;
+      const bool zf = owned && !first, xf = xsolver && xy, yf = ysolver && xy;
+      float fz[9], sz[9], fx[9], sx[9], fy[9], sy[9];
+      int qz = -1, qx = -1, qy = -1;
+      if (zf) {
+        float xin[SURR_IN];
+        const bool sm = surr_face_begin<2>(Lzp, Rz, P, thr, true, fz, sz, xin);
+        qz = surr_enqueue(sm, xin, Q);
+      }
+      if (xf) {
+        float L[9], xin[SURR_IN];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
+        const bool sm = surr_face_begin<0>(L, Rx, P, thr, true, fx, sx, xin);
+        qx = surr_enqueue(sm, xin, Q);
+      }
+      if (yf) {
+        float L[9], xin[SURR_IN];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
+        const bool sm = surr_face_begin<1>(L, Ry, P, thr, true, fy, sy, xin);
+        qy = surr_enqueue(sm, xin, Q);
+      }
+      __syncthreads();
+      const int nq = (int)Q.n;
+      for (int b = (t >> 6) * 32; b < nq; b += FTHREADS / 2) surr_batch(b, nq, Q, sW, sB);
+      __syncthreads();
+      float y[SURR_OUT];
+      if (owned) {
+        T Fz[9];
+        if (zf) {
+#pragma unroll
+          for (int k = 0; k < SURR_OUT; ++k) y[k] = qz >= 0 ? Q.y[qz * SURR_OUT + k] : 0.f;
+          surr_face_end<2>(fz, sz, y, qz >= 0, Fz);
+          // counts each face once: the lower faces of the stored cells
+          smooth_count += qz >= 0 && stores && !last;
+        }
+        zface_update(Fz, Lzn, m, first);
+      }
+      if (xy) {
+        if (xf) {
+          T Fx[9];
+#pragma unroll
+          for (int k = 0; k < SURR_OUT; ++k) y[k] = qx >= 0 ? Q.y[qx * SURR_OUT + k] : 0.f;
+          surr_face_end<0>(fx, sx, y, qx >= 0, Fx);
+          smooth_count += qx >= 0 && stores;
+#pragma unroll
+          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
+        }
+        if (yf) {
+          T Fy[9];
+#pragma unroll
+          for (int k = 0; k < SURR_OUT; ++k) y[k] = qy >= 0 ? Q.y[qy * SURR_OUT + k] : 0.f;
+          surr_face_end<1>(fy, sy, y, qy >= 0, Fy);
+          smooth_count += qy >= 0 && stores;
+#pragma unroll
+          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + iy * TX + ix] = Fy[v];
+        }
+        __syncthreads();
+        if (owned) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v)
+            Dxy[v] = (S.sLX[v * FNFX + iy * (TX + 1) + ix] - S.sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
+                     (S.sLY[v * FNFY + iy * TX + ix] - S.sLY[v * FNFY + (iy + 1) * TX + ix]);
+        }
+      }
+    } else if (xy) {
       T Fx[9], Fy[9];
-      if constexpr (SURR) {
-        T L[9], Rr[9];
-        if (xsolver) {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) {
-            L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
-            Rr[v] = Rx[v];
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) L[v] = Rr[v] = (v == 0 || v == 4) ? T(1) : T(0);
-        }
-        smooth_count += surr_face<0>(L, Rr, Fx, P, thr, xsolver, sW, sB) && stores;
-        if (ysolver) {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) {
-            L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
-            Rr[v] = Ry[v];
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) L[v] = Rr[v] = (v == 0 || v == 4) ? T(1) : T(0);
-        }
-        smooth_count += surr_face<1>(L, Rr, Fy, P, thr, ysolver, sW, sB) && stores;
-      } else {
+      {
       // masked fallback: LLF on every face touching a flagged cell (integrator_cube.jl:68-80)
       if (xsolver) {
         T L[9];
@@ -460,6 +504,7 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
   __shared__ float sB[SURR ? SIMG_NB : 1];
+  __shared__ __attribute__((aligned(16))) SurrQueue<SURR ? FQCAP : 1> sQ;
   unsigned smooth_count = 0;
   if constexpr (SURR) {
     const unsigned* src = reinterpret_cast<const unsigned*>(simg);
@@ -473,7 +518,9 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
-  const T smax = fused_item<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src);
+  SurrQueue<FQCAP>* q = nullptr;
+  if constexpr (SURR) q = &sQ;
+  const T smax = fused_item<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src, q);
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
   block_store_max(smax, sMax, red + blockIdx.x);
   if constexpr (SURR) {

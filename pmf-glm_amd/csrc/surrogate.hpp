@@ -170,11 +170,12 @@ __device__ __forceinline__ sf32x16 bias_acc(const float* __restrict__ b, int mb,
 // of a lane are exactly its B fragment for step s of the next layer (lane half h then
 // holds units 16s + 4h + {0..3} and 16s + 8 + 4h + {0..3}); the weight fragments read
 // the same units. Activations never leave registers.
+// npass = 1 evaluates only the faces of lanes 0-31 (results in lanes 0-31).
 __device__ __forceinline__ void mlp_wave(const float* x, float* y, const unsigned short* __restrict__ sw,
-                                         const float* __restrict__ sb) {
+                                         const float* __restrict__ sb, int npass = 2) {
   const int lane = threadIdx.x & 63, li = lane & 31, lh = lane >> 5;
 #pragma unroll 1
-  for (int pass = 0; pass < 2; ++pass) {
+  for (int pass = 0; pass < npass; ++pass) {
     const int src = pass * 32 + li;
     // layer 1: inputs 8h..8h+7 of face (pass, li) in lane half h
     bf16x8 bin;
@@ -274,6 +275,120 @@ __device__ __forceinline__ bool surr_face(const float* Lq, const float* Rq, floa
   F[5 + B] = f[7];
   F[8] = f[8];
   return smooth;
+}
+
+}  // namespace pmf
+
+namespace pmf {
+
+// The gated face flux in two halves around a batched network pass (the surrogate-assisted
+// fused step queues the smooth faces of a whole tile plane and evaluates them in dense
+// batches of 64, so waves whose faces are mostly rough do not run the network).
+// surr_face_begin: lab-frame states in; in the normal frame, f is the exact flux of a
+// rough face (HLLD, or LLF under the low-density switch) or the central flux of a smooth
+// one with the exact GLM slots, s the output scales and x the 16 network inputs. Returns
+// whether the face is smooth. Lanes with active false compute nothing of use.
+template <int D>
+__device__ __forceinline__ bool surr_face_begin(const float* Lq, const float* Rq, const StepParams<float>& P,
+                                                float thr, bool active, float* f, float* s, float* x) {
+  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
+  float L[9] = {Lq[0], Lq[1 + N], Lq[1 + A], Lq[1 + B], Lq[4], Lq[5 + N], Lq[5 + A], Lq[5 + B], Lq[8]};
+  float R[9] = {Rq[0], Rq[1 + N], Rq[1 + A], Rq[1 + B], Rq[4], Rq[5 + N], Rq[5 + A], Rq[5 + B], Rq[8]};
+  float fbn, fpsi;
+  const bool uself = surr_prepare(L, R, P, fbn, fpsi);
+  const bool smooth = surr_encode(L, R, P.gamma, thr, x, s) && !uself && active;
+  const bool rough = active && !smooth;
+  if (rough) {
+    if (uself) llf_n(L, R, P.ch, fbn, fpsi, f, P);
+    else hlld_n(L, R, L[5], fbn, fpsi, f, P);
+  }
+  if (smooth) {
+    surr_central(L, R, P.igm1, f);
+    f[5] = fbn;
+    f[8] = fpsi;
+  }
+  return smooth;
+}
+
+// surr_face_end: adds the learned dissipation y (smooth faces) and rotates to the lab frame.
+template <int D>
+__device__ __forceinline__ void surr_face_end(float* f, const float* s, const float* y, bool smooth, float* F) {
+  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
+  if (smooth) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      if (k != 5 && k != 8) f[k] = f[k] + s[k] * y[k];
+  }
+  F[0] = f[0];
+  F[1 + N] = f[1];
+  F[1 + A] = f[2];
+  F[1 + B] = f[3];
+  F[4] = f[4];
+  F[5 + N] = f[5];
+  F[5 + A] = f[6];
+  F[5 + B] = f[7];
+  F[8] = f[8];
+}
+
+// LDS queue of the smooth faces of one tile plane: inputs as bfloat16 (the precision the
+// network reads them in), outputs in float32.
+template <int CAP>
+struct SurrQueue {
+  unsigned n;
+  unsigned pad[3];
+  unsigned short x[CAP * SURR_IN];
+  float y[CAP * SURR_OUT];
+};
+
+// Appends the calling lanes' faces that want a slot (one LDS atomic per wave); returns
+// the slot, or -1.
+template <int CAP>
+__device__ __forceinline__ int surr_enqueue(bool want, const float* x, SurrQueue<CAP>& Q) {
+  const unsigned long long b = __ballot(want);
+  if (b == 0ull) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(b);
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(&Q.n, (unsigned)__popcll(b));
+  base = __shfl(base, leader, 64);
+  if (!want) return -1;
+  const int slot = int(base) + __popcll(b & ((1ull << lane) - 1ull));
+#pragma unroll
+  for (int k = 0; k < SURR_IN; k += 2) {
+    const unsigned w = (unsigned)bf16_bits(x[k]) | ((unsigned)bf16_bits(x[k + 1]) << 16);
+    *reinterpret_cast<unsigned*>(&Q.x[slot * SURR_IN + k]) = w;
+  }
+  return slot;
+}
+
+// One batch of up to 32 queued faces (slots b .. b+31 below nq, one MFMA pass), evaluated
+// by the calling wave (lanes 0-31 hold the faces); all lanes of the wave must call it.
+template <int CAP>
+__device__ __forceinline__ void surr_batch(int b, int nq, SurrQueue<CAP>& Q, const unsigned short* __restrict__ sw,
+                                           const float* __restrict__ sb) {
+  const int lane = threadIdx.x & 63;
+  const int slot = b + lane;
+  const bool valid = lane < 32 && slot < nq;
+  float x[SURR_IN];
+  if (valid) {
+    const uint4 a = *reinterpret_cast<const uint4*>(&Q.x[slot * SURR_IN]);
+    const uint4 c = *reinterpret_cast<const uint4*>(&Q.x[slot * SURR_IN + 8]);
+    const unsigned w[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      x[2 * k] = __builtin_bit_cast(float, w[k] << 16);
+      x[2 * k + 1] = __builtin_bit_cast(float, w[k] & 0xffff0000u);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SURR_IN; ++k) x[k] = 0.f;
+  }
+  float y[SURR_OUT];
+  mlp_wave(x, y, sw, sb, 1);
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < SURR_OUT; ++k) Q.y[slot * SURR_OUT + k] = y[k];
+  }
 }
 
 }  // namespace pmf
