@@ -105,14 +105,20 @@ __device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
 
 __device__ __forceinline__ int pidx(int ix, int iy) { return (iy + 2) * FPW + (ix + 2); }
 __device__ __forceinline__ int qidx(int ix, int iy) { return (iy + 1) * FQW + (ix + 1); }
+// Left-state slots of the x faces (cells ix = -1..TX-1 of the owned rows) and of the y
+// faces (cells iy = -1..TY-1 of the owned columns); the slot of a face's left cell is
+// also the slot of that face's flux.
+__device__ __forceinline__ int xlidx(int ix, int iy) { return iy * (TX + 1) + (ix + 1); }
+__device__ __forceinline__ int ylidx(int ix, int iy) { return (iy + 1) * TX + ix; }
 
-// LDS of one workgroup of the fused step.
-template <class T>
+// LDS of one workgroup of the fused step (37.9 KB without sources: four workgroups fit
+// a CU's 160 KB).
+template <class T, bool SRC = false>
 struct FusedLds {
   T sP[2][9 * FNP];  // primitive planes m (current) and m-1, alternating
-  T sLX[9 * FNQ];    // +x face states, then x-face fluxes
-  T sLY[9 * FNQ];    // +y face states, then y-face fluxes
-  unsigned char sMk[FNQ];  // masked-fallback flags of the predicted cells (sources only)
+  T sLX[9 * FNFX];   // left states of the x faces, then the x-face fluxes
+  T sLY[9 * FNFY];   // left states of the y faces, then the y-face fluxes
+  unsigned char sMk[SRC ? FNQ : 1];  // masked-fallback flags of the predicted cells
 };
 
 // One work item: tile w (x fastest, then y, then z chunk) of the grid g, marched through
@@ -124,7 +130,7 @@ struct FusedLds {
 // the lower face of its cell).
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
-                                        const StepParams<T>& P, int w, FusedLds<T>& S,
+                                        const StepParams<T>& P, int w, FusedLds<T, SRC>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
                                         SurrQueue<FQCAP>* Qp = nullptr) {
@@ -146,6 +152,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   const int64_t ecol = has_extra ? (int64_t)wrapi(y0 + ey, g.ny) * g.nx + wrapi(x0 + ex, g.nx) : 0;
   const bool xsolver = has_col && iy >= 0 && iy < TY && ix >= 0;  // owned + right ring column
   const bool ysolver = has_col && ix >= 0 && ix < TX && iy >= 0;  // owned + top ring row
+  const bool xleft = has_col && ix < TX && iy >= 0 && iy < TY;     // owned + left ring column
+  const bool yleft = has_col && iy < TY && ix >= 0 && ix < TX;     // owned + bottom ring row
   const int me = pidx(ix, iy);
 
   // Column registers: Lzp = left state of the z face below the current plane, Fzp = flux
@@ -313,16 +321,25 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         src_face(Ry, m0, dLy, src, pcell);
         src_face(Rz, m0, dLz, src, pcell);
         src_face(Lzn, m0, dRz, src, pcell);
+        if (xleft) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
-          S.sLX[v * FNQ + qidx(ix, iy)] = fx[v];
-          S.sLY[v * FNQ + qidx(ix, iy)] = fy[v];
+          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix, iy)] = fx[v];
+        }
+        if (yleft) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy)] = fy[v];
         }
       } else {
+        if (xleft) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix, iy)] = mh[v] + dRx[v];
+        }
+        if (yleft) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy)] = mh[v] + dRy[v];
+        }
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
-          S.sLX[v * FNQ + qidx(ix, iy)] = mh[v] + dRx[v];
-          S.sLY[v * FNQ + qidx(ix, iy)] = mh[v] + dRy[v];
           Rx[v] = mh[v] + dLx[v];
           Ry[v] = mh[v] + dLy[v];
           Rz[v] = mh[v] + dLz[v];
@@ -370,14 +387,14 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (xf) {
         float L[9], xin[SURR_IN];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
         const bool sm = surr_face_begin<0>(L, Rx, P, thr, true, fx, sx, xin);
         qx = surr_enqueue(sm, xin, Q);
       }
       if (yf) {
         float L[9], xin[SURR_IN];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
         const bool sm = surr_face_begin<1>(L, Ry, P, thr, true, fy, sy, xin);
         qy = surr_enqueue(sm, xin, Q);
       }
@@ -405,7 +422,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           surr_face_end<0>(fx, sx, y, qx >= 0, Fx);
           smooth_count += qx >= 0 && stores;
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
+          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix - 1, iy)] = Fx[v];
         }
         if (yf) {
           T Fy[9];
@@ -414,14 +431,14 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           surr_face_end<1>(fy, sy, y, qy >= 0, Fy);
           smooth_count += qy >= 0 && stores;
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + iy * TX + ix] = Fy[v];
+          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy - 1)] = Fy[v];
         }
         __syncthreads();
         if (owned) {
 #pragma unroll
           for (int v = 0; v < 9; ++v)
-            Dxy[v] = (S.sLX[v * FNFX + iy * (TX + 1) + ix] - S.sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
-                     (S.sLY[v * FNFY + iy * TX + ix] - S.sLY[v * FNFY + (iy + 1) * TX + ix]);
+            Dxy[v] = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
+                     (S.sLY[v * FNFY + ylidx(ix, iy - 1)] - S.sLY[v * FNFY + ylidx(ix, iy)]);
         }
       }
     } else if (xy) {
@@ -431,7 +448,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (xsolver) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNQ + qidx(ix - 1, iy)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
         if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)]))
           face_flux<0, false, T, RS_LLF>(L, Rx, Fx, P);
         else
@@ -440,7 +457,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (ysolver) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNQ + qidx(ix, iy - 1)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
         if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)]))
           face_flux<1, false, T, RS_LLF>(L, Ry, Fy, P);
         else
@@ -450,18 +467,18 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       __syncthreads();
       if (xsolver) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + iy * (TX + 1) + ix] = Fx[v];
+        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix - 1, iy)] = Fx[v];
       }
       if (ysolver) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + iy * TX + ix] = Fy[v];
+        for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy - 1)] = Fy[v];
       }
       __syncthreads();
       if (owned) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
-          Dxy[v] = (S.sLX[v * FNFX + iy * (TX + 1) + ix] - S.sLX[v * FNFX + iy * (TX + 1) + ix + 1]) +
-                   (S.sLY[v * FNFY + iy * TX + ix] - S.sLY[v * FNFY + (iy + 1) * TX + ix]);
+          Dxy[v] = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
+                   (S.sLY[v * FNFY + ylidx(ix, iy - 1)] - S.sLY[v * FNFY + ylidx(ix, iy)]);
       }
     }
   }
@@ -500,7 +517,7 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
                                                 const StepParams<T>& P, double* __restrict__ red,
                                                 const unsigned char* __restrict__ simg, float thr,
                                                 unsigned long long* __restrict__ nsmooth, const SrcParams<T>& src) {
-  __shared__ FusedLds<T> lds;
+  __shared__ FusedLds<T, SRC> lds;
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
   __shared__ float sB[SURR ? SIMG_NB : 1];
