@@ -129,6 +129,12 @@ int pmf_set_chunk(pmf_state* s, int32_t zc);
 /* Times the fused kernel for z chunks of 4..128 planes (reps launches each, writing
  * only the scratch fields) and keeps the fastest; returns it and its time per step. */
 int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
+/* Kernel of the CT (:ref) step. By default a constrained-transport state whose grid the
+ * fused kernel supports (nx a multiple of 16) steps on the fused z-marching kernel, which
+ * solves every face once with the :ref algorithm (integrator_ref.jl:83-139); per_cell != 0
+ * selects the per-cell kernel of integrator_ref.jl:189-220 instead. No counterpart in the
+ * reference (one kernel there); results agree to rounding. */
+int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell);
 /* Kernel timing: while enabled, HIP events bracket every step-kernel launch on the
  * state's stream; pmf_timing_read waits for the stream and returns the summed kernel
  * time (ms) and the number of timed launches since the last enable. */

@@ -64,6 +64,11 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
         launch_step_fused<T>(in, out, s->G, make_params<T>(s, dt, ch, decay), s->red, s->dval, s->st, s->zc);
     mark(s);
     if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
+  } else if (ct && !s->ct_percell && fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann)) {
+    // the same :ref algorithm with each face solved once (z-marching tiles)
+    mark(s);
+    launch_step_fused_ct<T>(in, out, ctp, s->G, make_params<T>(s, dt, 0.0, 1.0), s->st, s->zc);
+    mark(s);
   } else {
     mark(s);
     launch_step_ref<T>(in, out, ctp, s->G, make_params<T>(s, dt, ct ? 0.0 : ch, ct ? 1.0 : decay), ct, s->st);
@@ -401,6 +406,13 @@ int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax) {
       *dt = double(float(cfl) * float(s->p.dx) / std::max(smf, std::numeric_limits<float>::epsilon()));
     }
     if (smax) *smax = sm;
+  });
+}
+
+int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell) {
+  return guard([&] {
+    check_state(s);
+    s->ct_percell = per_cell != 0;
   });
 }
 

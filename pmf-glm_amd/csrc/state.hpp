@@ -53,6 +53,7 @@ struct pmf_state {
   int device = 0;
   bool smax_valid = false;  // dval[0] holds the CFL speed of the current U
   int zc = 0;               // planes per z chunk of the fused kernel (0 = built-in choice)
+  bool ct_percell = false;  // CT steps on the per-cell kernel even where the fused one applies
   bool timing = false;      // bracket step kernels with events
   std::vector<hipEvent_t> ev;  // pairs (start, stop) in launch order
   size_t ev_used = 0;

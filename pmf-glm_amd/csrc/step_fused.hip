@@ -128,12 +128,13 @@ struct FusedLds {
 // learned flux (wave-level MFMA) or by HLLD; sW/sB hold the network's LDS image
 // (surrogate.hpp) and smooth_count counts the faces that used it (each face once, as
 // the lower face of its cell).
-template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T, SRC>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
-                                        SurrQueue<FQCAP>* Qp = nullptr) {
+                                        SurrQueue<FQCAP>* Qp = nullptr, T* __restrict__ ctp = nullptr) {
+  static_assert(!(CT && (SURR || SRC)), "the fused CT step has no surrogate or source terms");
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -203,10 +204,16 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
       }
-      o[8] *= P.decay;
+      if constexpr (CT) {
+        o[8] = T(0);  // no cleaning field under CT (integrator_ref.jl:150-158)
+      } else {
+        o[8] *= P.decay;
+      }
 #pragma unroll
       for (int v = 0; v < 9; ++v) *cell_at<NARROW>(O.v[v], idx) = o[v];
-      smax = nanmax(smax, cell_wavespeed(o, P));
+      // CT: the cell field is replaced by the face average after the step, so the CFL
+      // speed is taken from the final state instead
+      if constexpr (!CT) smax = nanmax(smax, cell_wavespeed(o, P));
     }
     if (!first) {
 #pragma unroll
@@ -347,14 +354,41 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) Lzn[v] = mh[v] + dRz[v];
       }
+      if constexpr (CT) {
+        // cell EMF -v x B from the predicted velocity and the start-of-step field
+        // (integrator_ref.jl:133-138), for the chunk's own cells
+        if (stores && !first && !last) {
+          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
+          ctp[12 * n + idx] = mh[3] * m0[6] - mh[2] * m0[7];
+          ctp[13 * n + idx] = mh[1] * m0[7] - mh[3] * m0[5];
+          ctp[14 * n + idx] = mh[2] * m0[5] - mh[1] * m0[6];
+        }
+      }
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
       // whose inputs all exist now (integrator_ref.jl:140-158)
       if constexpr (!SURR) {
         if (owned) {
           T Fz[9];
           if (!first) {
-            if (SRC && src.check == CHECK_MASKED && (mkz || mkzp)) face_flux<2, false, T, RS_LLF>(Lzp, Rz, Fz, P);
-            else face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+            if constexpr (CT) {
+              // riemann_ct on the z face m-1/2 = the lower face of plane m: the face field
+              // in both states; its mass and transverse-B fluxes go to the workspace
+              // (integrator_ref.jl:83-132) when plane m belongs to this chunk
+              const int64_t idx = (int64_t)wrapi(m, g.nz) * g.plane + col, n = g.nc;
+              const T bf = ctp[2 * n + idx];
+              Lzp[7] = bf;
+              Rz[7] = bf;
+              face_flux<2, true, T, RS>(Lzp, Rz, Fz, P);
+              if (stores && !last) {
+                ctp[5 * n + idx] = Fz[0];
+                ctp[10 * n + idx] = Fz[5];
+                ctp[11 * n + idx] = Fz[6];
+              }
+            } else if (SRC && src.check == CHECK_MASKED && (mkz || mkzp)) {
+              face_flux<2, false, T, RS_LLF>(Lzp, Rz, Fz, P);
+            } else {
+              face_flux<2, false, T, RS>(Lzp, Rz, Fz, P);
+            }
           }
           zface_update(Fz, Lzn, m, first);
         }
@@ -449,19 +483,43 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         T L[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
-        if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)]))
+        if constexpr (CT) {
+          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
+          const T bf = ctp[idx];
+          L[5] = bf;
+          Rx[5] = bf;
+          face_flux<0, true, T, RS>(L, Rx, Fx, P);
+          if (stores) {
+            ctp[3 * n + idx] = Fx[0];
+            ctp[6 * n + idx] = Fx[6];
+            ctp[7 * n + idx] = Fx[7];
+          }
+        } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)])) {
           face_flux<0, false, T, RS_LLF>(L, Rx, Fx, P);
-        else
+        } else {
           face_flux<0, false, T, RS>(L, Rx, Fx, P);
+        }
       }
       if (ysolver) {
         T L[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
-        if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)]))
+        if constexpr (CT) {
+          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
+          const T bf = ctp[n + idx];
+          L[6] = bf;
+          Ry[6] = bf;
+          face_flux<1, true, T, RS>(L, Ry, Fy, P);
+          if (stores) {
+            ctp[4 * n + idx] = Fy[0];
+            ctp[8 * n + idx] = Fy[5];
+            ctp[9 * n + idx] = Fy[7];
+          }
+        } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)])) {
           face_flux<1, false, T, RS_LLF>(L, Ry, Fy, P);
-        else
+        } else {
           face_flux<1, false, T, RS>(L, Ry, Fy, P);
+        }
       }
       }
       __syncthreads();
@@ -859,6 +917,51 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 #undef PMF_ENS_RS
 #undef PMF_ENS
 }
+
+// The fused CT step: the :ref algorithm for constrained-transport states (riemann_ct on
+// the face fields, integrator_ref.jl:83-139) with each face solved once; writes the cell
+// update and workspace slots 3-14 (face mass and transverse-B fluxes of every cell's
+// lower faces, cell EMFs) that ct_update! reads.
+template <class T, int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) void k_step_fused_ct(Fields<const T> U, Fields<T> O, T* __restrict__ ctp,
+                                                            FusedGeom g, StepParams<T> P) {
+  __shared__ FusedLds<T> lds;
+  const int nwg = gridDim.x;
+  int w = blockIdx.x;
+  if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
+  unsigned none = 0;
+  const SrcParams<T> nosrc{};
+  (void)fused_item<T, RECON, RS, false, false, false, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, nosrc,
+                                                            nullptr, ctp);
+}
+
+template <class T>
+bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
+                          hipStream_t st, int zc) {
+  if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
+  FusedGeom g = fused_geom(G, chunk_of(G, zc));
+  dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
+#define PMF_CT(R, S) hipLaunchKernelGGL((k_step_fused_ct<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P)
+#define PMF_CT_RS(R)                                \
+  if (P.rsolve == RS_HLL) PMF_CT(R, RS_HLL);        \
+  else if (P.rsolve == RS_LLF) PMF_CT(R, RS_LLF);   \
+  else PMF_CT(R, RS_HLLD);
+  if (P.recon == RECON_PPM) {
+    PMF_CT_RS(RECON_PPM)
+  } else if (P.recon == RECON_PLM_MINMOD) {
+    PMF_CT_RS(RECON_PLM_MINMOD)
+  } else {
+    PMF_CT_RS(RECON_PLM)
+  }
+#undef PMF_CT_RS
+#undef PMF_CT
+  return true;
+}
+
+template bool launch_step_fused_ct<float>(Fields<const float>, Fields<float>, float*, const Grid&,
+                                          const StepParams<float>&, hipStream_t, int);
+template bool launch_step_fused_ct<double>(Fields<const double>, Fields<double>, double*, const Grid&,
+                                           const StepParams<double>&, hipStream_t, int);
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
                                        double*, double*, hipStream_t, int, const SrcParams<float>*);

@@ -331,6 +331,12 @@ def set_chunk(s: MHDState, zc: int):
     check(lib().pmf_set_chunk(s.handle, int(zc)))
 
 
+def set_ct_kernel(s: MHDState, per_cell: bool):
+    """CT steps on the per-cell kernel (True) or, where the grid allows (nx a multiple
+    of 16), on the fused kernel that solves every face once (False, the default)."""
+    check(lib().pmf_set_ct_kernel(s.handle, 1 if per_cell else 0))
+
+
 def autotune(s: MHDState, reps: int = 5):
     """Time the fused kernel over z-chunk depths on this state (scratch is overwritten) and
     keep the fastest; returns (zc, ms per step)."""

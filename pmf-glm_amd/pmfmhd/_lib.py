@@ -71,6 +71,7 @@ def lib():
             "pmf_surrogate_face_flux": [P, i64, i64, i32, d, P],
             "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
             "pmf_set_chunk": [P, i32],
+            "pmf_set_ct_kernel": [P, i32],
             "pmf_step_drag": [P, d, d, d, d, pd, i32],
             "pmf_buffer_alloc": [i64, pp],
             "pmf_species_to_fractions": [P, P, P, P, d],

@@ -227,6 +227,66 @@ def test_ct_float32_divergence_preserved_and_faces_from_cells():
     assert max(divs) < 1e-5 and len(divs) > 5
 
 
+@pytest.mark.parametrize("recon", ["plm", "plm_minmod", "ppm"])
+@pytest.mark.parametrize("riemann", ["hlld", "hll", "llf"])
+def test_ct_fused_kernel_matches_oracle_and_per_cell_kernel_float64(recon, riemann):
+    """CT states step on the fused z-marching kernel where nx is a multiple of 16 (the
+    :ref algorithm with every face solved once): against the float64 oracle (cells and
+    faces) and against the per-cell kernel (every CT slot, the workspace included). 16^3
+    has a ragged last tile row and four z chunks."""
+    from test_oracle import ct_test_hydro, ct_test_potential
+    N = 16
+    kw = dict(riemann=riemann, recon=recon, mhd_solver="ct", smallr=F.FK_SMALLR, pfl=F.FK_SMALLP)
+    g = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float64, **kw)
+    p = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float64, **kw)
+    M.set_ct_kernel(p, True)
+    o = O.OracleState((N, N, N), 1.0 / N, **kw)
+    U = ct_test_hydro(N, G)
+    A = ct_test_potential(N)
+    for st in (g, p):
+        st.set_conserved(U)
+        M.ct_faces_from_potential(st, *A)
+    o.set_conserved(U)
+    o.ct_faces_from_potential(*A)
+    dt, _ = o.compute_dt(0.3)
+    for _ in range(3):
+        o.step(dt)
+        M.step(g, dt)
+        M.step(p, dt)
+    Ug, Up = g.conserved(), p.conserved()
+    for v in range(9):
+        scale = max(np.abs(o.U[v]).max(), 1e-6)
+        assert np.max(np.abs(Ug[v] - o.U[v])) <= 1e-12 * scale, v
+        assert np.max(np.abs(Ug[v] - Up[v])) <= 1e-12 * scale, v
+    for d in range(3):
+        assert np.max(np.abs(g.ct_slot(d) - o.ct[d])) <= 1e-12 * np.abs(o.ct[d]).max()
+    for q in range(21):
+        a, b = g.ct_slot(q), p.ct_slot(q)
+        assert np.max(np.abs(a - b)) <= 1e-12 * max(np.abs(b).max(), 1e-300), q
+    assert M.max_divb_face(g) / N < 1e-12
+    assert np.all(Ug[8] == 0)
+
+
+def test_ct_fused_kernel_float32_matches_per_cell_kernel():
+    """256^3-recipe turbulence at 32^3 (float32): the fused CT step against the per-cell
+    one after 5 CFL steps, and the face divergence stays at rounding level."""
+    N = 32
+    s = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, recon="plm_minmod", riemann="hlld", mhd_solver="ct")
+    p = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, recon="plm_minmod", riemann="hlld", mhd_solver="ct")
+    M.set_ct_kernel(p, True)
+    U = P.config_state("c2_256", seed=2, n=N).astype(np.float32)
+    for st in (s, p):
+        st.set_conserved(U)
+        M.ct_faces_from_cells(st)
+    for _ in range(5):
+        dt, _ = M.compute_dt(s, 0.3)
+        M.step(s, float(dt))
+        M.step(p, float(dt))
+    a, b = s.conserved().astype(np.float64), p.conserved().astype(np.float64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    assert M.max_divb_face(s) / N < 1e-5
+
+
 def test_errors_carry_reference_text():
     s = M.allocate_state((8, 8, 8), 1 / 8)
     with pytest.raises(M.MHDError, match="needs the GLM cleaning speed"):
