@@ -229,55 +229,6 @@ __device__ __forceinline__ void mlp_wave(const float* x, float* y, const unsigne
 
 }  // namespace pmf
 
-namespace pmf {
-
-// Gated face flux for the surrogate-assisted step: lab-frame states in, lab-frame flux
-// out. Must be called by all lanes of a wave (active marks lanes that own a face; the
-// others pass any finite state). Rough or inactive-adjacent faces take HLLD (LLF under
-// the low-density switch), smooth ones the central flux plus the learned dissipation.
-// Returns whether this lane's face used the network.
-template <int D>
-__device__ __forceinline__ bool surr_face(const float* Lq, const float* Rq, float* F, const StepParams<float>& P,
-                                          float thr, bool active, const unsigned short* sw, const float* sb) {
-  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
-  float L[9] = {Lq[0], Lq[1 + N], Lq[1 + A], Lq[1 + B], Lq[4], Lq[5 + N], Lq[5 + A], Lq[5 + B], Lq[8]};
-  float R[9] = {Rq[0], Rq[1 + N], Rq[1 + A], Rq[1 + B], Rq[4], Rq[5 + N], Rq[5 + A], Rq[5 + B], Rq[8]};
-  float fbn, fpsi;
-  const bool uself = surr_prepare(L, R, P, fbn, fpsi);
-  float x[16], s[9], f[9];
-  const bool smooth = surr_encode(L, R, P.gamma, thr, x, s) && !uself && active;
-  const bool rough = active && !smooth;
-  if (__any(rough)) {
-    if (rough) {
-      if (uself) llf_n(L, R, P.ch, fbn, fpsi, f, P);
-      else hlld_n(L, R, L[5], fbn, fpsi, f, P);
-    }
-  }
-  if (__any(smooth)) {
-    float y[9];
-    mlp_wave(x, y, sw, sb);
-    if (smooth) {
-      float fc[9];
-      surr_central(L, R, P.igm1, fc);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) f[k] = fc[k] + s[k] * y[k];
-      f[5] = fbn;
-      f[8] = fpsi;
-    }
-  }
-  F[0] = f[0];
-  F[1 + N] = f[1];
-  F[1 + A] = f[2];
-  F[1 + B] = f[3];
-  F[4] = f[4];
-  F[5 + N] = f[5];
-  F[5 + A] = f[6];
-  F[5 + B] = f[7];
-  F[8] = f[8];
-  return smooth;
-}
-
-}  // namespace pmf
 
 namespace pmf {
 
