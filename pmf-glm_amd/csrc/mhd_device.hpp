@@ -49,15 +49,22 @@ __device__ __forceinline__ double dmax(double a, double b) { return __builtin_fm
 __device__ __forceinline__ float dmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
 __device__ __forceinline__ float dabs(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ double dabs(double a) { return __builtin_fabs(a); }
 
 // cons2prim (glm_physics.jl:11-18): density floored before the velocity and kinetic
 // energy, pressure floored at pfl.
+// The fused step converts its LDS planes through several inlined copies of this
+// function (column cells, halo cells, chunk start), so its arithmetic is pinned: explicit
+// FMAs and no other contraction. A cell then gets the same primitives whichever copy
+// converts it, and the step's result does not depend on where the tile seams fall.
 template <class T> __device__ __forceinline__ void cons2prim(const T* c, T* q, const StepParams<T>& P) {
+#pragma clang fp contract(off)
   T rho = dmax(c[0], P.smallr);
   T ir = rcp(rho);
-  T ek = T(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
-  T em = T(0.5) * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
+  T ek = T(0.5) * dfma(c[3], c[3], dfma(c[2], c[2], c[1] * c[1])) * ir;
+  T em = T(0.5) * dfma(c[7], c[7], dfma(c[6], c[6], c[5] * c[5]));
   q[0] = rho;
   q[1] = c[1] * ir;
   q[2] = c[2] * ir;

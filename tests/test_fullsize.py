@@ -1,0 +1,77 @@
+"""Size-independent properties at the benchmark size (256^3 float32, the
+plain-turbulence workload): exact translation equivariance of the fused step,
+conservation of the periodic totals, and agreement of the fused kernels with the
+per-cell (:ref) kernels for GLM and CT states."""
+import numpy as np
+import pytest
+
+import pmfmhd as M
+from pmfmhd import problems as P
+
+pytestmark = pytest.mark.gpu
+N = 256
+
+
+@pytest.fixture(scope="module")
+def state256():
+    return P.config_state("c2_256", seed=1).astype(np.float32)
+
+
+def _state(U, **kw):
+    s = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, recon="plm_minmod", **kw)
+    s.set_conserved(U)
+    return s
+
+
+def test_fused_step_is_translation_equivariant_bitwise(state256):
+    """Every face is solved by the same arithmetic wherever it falls (tile interior, tile
+    ring, z-chunk seam), so a periodic shift of the input shifts the output exactly."""
+    U = state256
+    a = _state(U)
+    dt, sm = M.compute_dt(a, 0.2)
+    M.step(a, float(dt), ch=float(sm), integrator="cube")
+    shift = (3, 7, 5)  # cells along z, y, x: not multiples of the 12 x 16 tile or the chunk
+    Us = np.roll(U.reshape(9, N, N, N), shift, axis=(1, 2, 3)).reshape(9, -1)
+    b = _state(np.ascontiguousarray(Us))
+    M.step(b, float(dt), ch=float(sm), integrator="cube")
+    ra = np.roll(a.conserved().reshape(9, N, N, N), shift, axis=(1, 2, 3)).reshape(9, -1)
+    assert np.array_equal(ra, b.conserved())
+
+
+def test_fused_steps_conserve_the_periodic_totals(state256):
+    s = _state(state256)
+    t0 = M.conserved_totals(s)
+    for _ in range(5):
+        dt, sm = M.compute_dt(s, 0.2)
+        M.step(s, float(dt), ch=float(sm), integrator="cube")
+    t1 = M.conserved_totals(s)
+    for k in ("mass", "energy"):
+        assert abs(t1[k] - t0[k]) <= 1e-9 * abs(t0[k]), k
+    scale = np.sqrt(t0["energy"])  # momentum and field totals start near zero
+    for k in ("momx", "momy", "momz", "Bx", "By", "Bz"):
+        assert abs(t1[k] - t0[k]) <= 1e-9 * scale, k
+    assert all(np.isfinite(a).all() for a in s.fields_to_host())
+
+
+def test_fused_and_per_cell_glm_steps_agree(state256):
+    a, b = _state(state256), _state(state256)
+    dt, sm = M.compute_dt(a, 0.2)
+    M.step(a, float(dt), ch=float(sm), integrator="cube")
+    M.step(b, float(dt), ch=float(sm), integrator="ref")
+    x, y = a.conserved().astype(np.float64), b.conserved().astype(np.float64)
+    for v in range(8):
+        assert np.max(np.abs(x[v] - y[v])) <= 2e-6 * np.abs(y[v]).max(), v
+
+
+def test_fused_and_per_cell_ct_steps_agree(state256):
+    a, b = _state(state256, mhd_solver="ct"), _state(state256, mhd_solver="ct")
+    M.set_ct_kernel(b, True)
+    for s in (a, b):
+        M.ct_faces_from_cells(s)
+    dt, _ = M.compute_dt(a, 0.2)
+    for s in (a, b):
+        M.step(s, float(dt))
+    x, y = a.conserved().astype(np.float64), b.conserved().astype(np.float64)
+    for v in range(8):
+        assert np.max(np.abs(x[v] - y[v])) <= 2e-6 * np.abs(y[v]).max(), v
+    assert M.max_divb_face(a) / N < 1e-5
