@@ -8,7 +8,8 @@
 // faces (8 threads have no column). Two LDS planes hold the primitive states of planes
 // m and m-1 with both halo rings; the next plane enters the buffer of m-1 as soon as
 // each column's own slot there is no longer needed. With the kernel's register budget
-// a SIMD holds two waves, so two 4-wave workgroups fill a CU evenly. Per plane m:
+// (about 165 VGPRs) a SIMD holds three waves for the float minmod and MonCen-HLL/LLF
+// variants (two for the rest), so three (two) 4-wave workgroups share a CU. Per plane m:
 //   1. convert the next plane of the own column (and, for a few threads, of the extra
 //      halo cells) from conserved to primitive;
 //   2. Hancock-predict plane m in every column (cons2prim of the half-step state);
@@ -16,7 +17,7 @@
 //      and the z face m-1/2 of each owned column (both states in registers);
 //   4. update plane m-1 of the owned columns with the reference's flux grouping, damp
 //      psi, store, and fold the new cell's CFL speed into a running maximum.
-// Every cell is predicted once per tile pass (1.25x with the ring), every face is solved
+// Every cell is predicted once per tile pass (1.29x with the ring), every face is solved
 // once, and the global state is read once and written once per step. Workgroups are
 // remapped so that consecutive tiles land on the same XCD (and its L2), which then
 // serves the halo re-reads.
@@ -558,10 +559,8 @@ __device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
   }
 }
 
-// The single-grid step: one workgroup per work item. simg is the network's image in
-// global memory (SURR only) and nsmooth the count of network-solved faces.
 // Three waves per SIMD for the float variants whose registers fit 168 VGPRs without
-// spilling (three 41 KB workgroups then share a CU's LDS); the compiler's default
+// spilling (three 38 KB workgroups then share a CU's LDS); the compiler's default
 // (two waves) for the rest.
 template <class T, int RECON, int RS, bool SURR, bool SRC>
 constexpr bool fused_three_waves() {
