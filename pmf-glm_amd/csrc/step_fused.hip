@@ -126,9 +126,9 @@ struct FusedLds {
 // its z chunk. Called by all FTHREADS threads of the workgroup; returns the calling
 // thread's running CFL maximum of the cells it stored. SURR: the surrogate-assisted
 // variant (float only): each face is gated by the jump detector and solved by the
-// learned flux (wave-level MFMA) or by HLLD; sW/sB hold the network's LDS image
-// (surrogate.hpp) and smooth_count counts the faces that used it (each face once, as
-// the lower face of its cell).
+// learned flux (wave-level MFMA) or by HLLD; sW holds the network's LDS image
+// (surrogate.hpp), sB its biases (global memory), and smooth_count counts the faces that
+// used it (each face once, as the lower face of its cell).
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T, SRC>& S,
@@ -441,8 +441,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (owned) {
         T Fz[9];
         if (zf) {
-#pragma unroll
-          for (int k = 0; k < SURR_OUT; ++k) y[k] = qz >= 0 ? Q.y[qz * SURR_OUT + k] : 0.f;
+          surr_result(Q, qz, y);
           surr_face_end<2>(fz, sz, y, qz >= 0, Fz);
           // counts each face once: the lower faces of the stored cells
           smooth_count += qz >= 0 && stores && !last;
@@ -452,8 +451,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (xy) {
         if (xf) {
           T Fx[9];
-#pragma unroll
-          for (int k = 0; k < SURR_OUT; ++k) y[k] = qx >= 0 ? Q.y[qx * SURR_OUT + k] : 0.f;
+          surr_result(Q, qx, y);
           surr_face_end<0>(fx, sx, y, qx >= 0, Fx);
           smooth_count += qx >= 0 && stores;
 #pragma unroll
@@ -461,8 +459,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         }
         if (yf) {
           T Fy[9];
-#pragma unroll
-          for (int k = 0; k < SURR_OUT; ++k) y[k] = qy >= 0 ? Q.y[qy * SURR_OUT + k] : 0.f;
+          surr_result(Q, qy, y);
           surr_face_end<1>(fy, sy, y, qy >= 0, Fy);
           smooth_count += qy >= 0 && stores;
 #pragma unroll
@@ -577,15 +574,12 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   __shared__ FusedLds<T, SRC> lds;
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
-  __shared__ float sB[SURR ? SIMG_NB : 1];
   __shared__ __attribute__((aligned(16))) SurrQueue<SURR ? FQCAP : 1> sQ;
   unsigned smooth_count = 0;
   if constexpr (SURR) {
     const unsigned* src = reinterpret_cast<const unsigned*>(simg);
     unsigned* dw = reinterpret_cast<unsigned*>(sW);
     for (int e = threadIdx.x; e < SIMG_NW / 2; e += FTHREADS) dw[e] = src[e];
-    const float* bsrc = reinterpret_cast<const float*>(simg + SIMG_NW * 2);
-    for (int e = threadIdx.x; e < SIMG_NB; e += FTHREADS) sB[e] = bsrc[e];
   }
   // XCD-aware tile order: dispatch round-robins workgroups over the 8 XCDs, so
   // logical tile (w % 8) * (nwg / 8) + w / 8 gives each XCD a contiguous tile range.
@@ -594,6 +588,9 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
   SurrQueue<FQCAP>* q = nullptr;
   if constexpr (SURR) q = &sQ;
+  // the biases (the image's float tail) stay in global memory: with them out of LDS the
+  // surrogate step's 81.6 KB lets two workgroups share a CU
+  const float* sB = SURR ? reinterpret_cast<const float*>(simg + SIMG_NW * 2) : nullptr;
   const T smax = fused_item<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src, q);
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
   block_store_max(smax, sMax, red + blockIdx.x);
@@ -619,12 +616,25 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) v
   step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
+// The surrogate-assisted step: two waves per SIMD (about 250 VGPRs with the network in its
+// own function; 81.6 KB of LDS), so two workgroups share a CU and one's network batches
+// overlap the other's HLLD faces and barriers.
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_step_fused2(
+    Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
+    const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+  step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
+}
+
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
 void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                           const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
                           unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{}) {
   if constexpr (fused_three_waves<T, RECON, RS, SURR, SRC>())
     hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
+                       simg, thr, nsmooth, src);
+  else if constexpr (SURR)
+    hipLaunchKernelGGL((k_step_fused2<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
                        simg, thr, nsmooth, src);
   else
     hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,

@@ -171,8 +171,9 @@ __device__ __forceinline__ sf32x16 bias_acc(const float* __restrict__ b, int mb,
 // holds units 16s + 4h + {0..3} and 16s + 8 + 4h + {0..3}); the weight fragments read
 // the same units. Activations never leave registers.
 // npass = 1 evaluates only the faces of lanes 0-31 (results in lanes 0-31).
-__device__ __forceinline__ void mlp_wave(const float* x, float* y, const unsigned short* __restrict__ sw,
-                                         const float* __restrict__ sb, int npass = 2) {
+__device__ inline __attribute__((noinline)) void mlp_wave(const float* x, float* y,
+                                                        const unsigned short* __restrict__ sw,
+                                                        const float* __restrict__ sb, int npass = 2) {
   const int lane = threadIdx.x & 63, li = lane & 31, lh = lane >> 5;
 #pragma unroll 1
   for (int pass = 0; pass < npass; ++pass) {
@@ -281,14 +282,18 @@ __device__ __forceinline__ void surr_face_end(float* f, const float* s, const fl
   F[8] = f[8];
 }
 
-// LDS queue of the smooth faces of one tile plane: inputs as bfloat16 (the precision the
-// network reads them in), outputs in float32.
+// LDS queue of the smooth faces of one tile plane: one 32-byte slot per face holding its
+// 16 network inputs as bfloat16 (the precision the network reads them in), which the
+// network pass overwrites with the seven learned outputs in float32 (the GLM slots 5 and
+// 8 are exact and not learned).
+constexpr int SURR_QOUT = 7;
+__device__ __forceinline__ constexpr int surr_qk(int j) { return j < 5 ? j : j + 1; }  // output slot j -> k
+
 template <int CAP>
 struct SurrQueue {
   unsigned n;
   unsigned pad[3];
-  unsigned short x[CAP * SURR_IN];
-  float y[CAP * SURR_OUT];
+  uint4 slot[CAP * 2];  // 32 bytes per face
 };
 
 // Appends the calling lanes' faces that want a slot (one LDS atomic per wave); returns
@@ -304,11 +309,11 @@ __device__ __forceinline__ int surr_enqueue(bool want, const float* x, SurrQueue
   base = __shfl(base, leader, 64);
   if (!want) return -1;
   const int slot = int(base) + __popcll(b & ((1ull << lane) - 1ull));
+  unsigned w[8];
 #pragma unroll
-  for (int k = 0; k < SURR_IN; k += 2) {
-    const unsigned w = (unsigned)bf16_bits(x[k]) | ((unsigned)bf16_bits(x[k + 1]) << 16);
-    *reinterpret_cast<unsigned*>(&Q.x[slot * SURR_IN + k]) = w;
-  }
+  for (int k = 0; k < 8; ++k) w[k] = (unsigned)bf16_bits(x[2 * k]) | ((unsigned)bf16_bits(x[2 * k + 1]) << 16);
+  Q.slot[2 * slot] = make_uint4(w[0], w[1], w[2], w[3]);
+  Q.slot[2 * slot + 1] = make_uint4(w[4], w[5], w[6], w[7]);
   return slot;
 }
 
@@ -322,8 +327,8 @@ __device__ __forceinline__ void surr_batch(int b, int nq, SurrQueue<CAP>& Q, con
   const bool valid = lane < 32 && slot < nq;
   float x[SURR_IN];
   if (valid) {
-    const uint4 a = *reinterpret_cast<const uint4*>(&Q.x[slot * SURR_IN]);
-    const uint4 c = *reinterpret_cast<const uint4*>(&Q.x[slot * SURR_IN + 8]);
+    const uint4 a = Q.slot[2 * slot];
+    const uint4 c = Q.slot[2 * slot + 1];
     const unsigned w[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -336,9 +341,23 @@ __device__ __forceinline__ void surr_batch(int b, int nq, SurrQueue<CAP>& Q, con
   }
   float y[SURR_OUT];
   mlp_wave(x, y, sw, sb, 1);
+  // the slot's inputs were read above by this lane, the only one that touches the slot
   if (valid) {
+    float* out = reinterpret_cast<float*>(&Q.slot[2 * slot]);
 #pragma unroll
-    for (int k = 0; k < SURR_OUT; ++k) Q.y[slot * SURR_OUT + k] = y[k];
+    for (int j = 0; j < SURR_QOUT; ++j) out[j] = y[surr_qk(j)];
+  }
+}
+
+// The learned outputs of queue slot q (zeros for q < 0), after the network pass.
+template <int CAP>
+__device__ __forceinline__ void surr_result(const SurrQueue<CAP>& Q, int q, float* y) {
+#pragma unroll
+  for (int k = 0; k < SURR_OUT; ++k) y[k] = 0.f;
+  if (q >= 0) {
+    const float* in = reinterpret_cast<const float*>(&Q.slot[2 * q]);
+#pragma unroll
+    for (int j = 0; j < SURR_QOUT; ++j) y[surr_qk(j)] = in[j];
   }
 }
 
