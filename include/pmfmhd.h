@@ -126,8 +126,10 @@ int pmf_riemann_flux_batch(pmf_state* s, const void* L, const void* R, int32_t d
 /* Tile depth of the fused (:cube) kernel: planes per z chunk, at least 4 (0 = the
  * built-in choice, the default). Results do not depend on it. */
 int pmf_set_chunk(pmf_state* s, int32_t zc);
-/* Times the fused kernel for z chunks of 4..128 planes (reps launches each, writing
- * only the scratch fields) and keeps the fastest; returns it and its time per step. */
+/* Times the fused kernel (the plain step, or the fused CT step for CT states) for z
+ * chunks of 4..128 planes (reps launches each, writing only the scratch fields and, for
+ * CT, the per-step workspace slots) and keeps the fastest; returns it and its time per
+ * step. */
 int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
 /* Kernel of the CT (:ref) step. By default a constrained-transport state whose grid the
  * fused kernel supports (nx a multiple of 16) steps on the fused z-marching kernel, which

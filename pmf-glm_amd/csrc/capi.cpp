@@ -484,8 +484,10 @@ int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out) {
   return guard([&] {
     check_fields(s);
     if (reps < 1) throw Error("reps must be positive");
-    if (s->p.solver != PMF_SOLVER_GLM || !fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
+    if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
       throw Error("autotune: the fused kernel does not run on this state");
+    const bool ct = s->p.solver == PMF_SOLVER_CT;
+    if (ct && !s->ct) throw Error("ct_update! needs a CT state");
     double dt, smax;
     if (pmf_compute_dt(s, 0.2, &dt, &smax)) throw Error(g_err);
     const double ch = smax > 0 ? smax : 1.0;
@@ -499,13 +501,22 @@ int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out) {
       for (int zc : {4, 8, 12, 16, 24, 32, 48, 64, 96, 128}) {
         if (zc > s->G.nz && zc != 4) break;
         s->zc = zc;
+        // writes only the scratch fields (and, for CT, workspace slots 3-14, which every
+        // step recomputes): the state and its CT faces are untouched
         auto launch = [&]() {
-          if (s->p.precision == PMF_F64)
+          if (ct && s->p.precision == PMF_F64)
+            launch_step_fused_ct<double>(cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc),
+                                         (double*)s->ct, s->G, make_params<double>(s, dt, 0.0, 1.0), s->st, s->zc);
+          else if (ct)
+            launch_step_fused_ct<float>(cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc),
+                                        (float*)s->ct, s->G, make_params<float>(s, dt, 0.0, 1.0), s->st, s->zc);
+          else if (s->p.precision == PMF_F64)
             step_fields<double>(s, cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc), nullptr, dt, ch,
                                 1.0, PMF_INTEGRATOR_CUBE);
           else
             step_fields<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), nullptr, dt, ch, 1.0,
                                PMF_INTEGRATOR_CUBE);
+          hip_check(hipGetLastError(), "autotune launch");
         };
         launch();  // warm-up
         hip_check(hipEventRecord(e0, s->st), "hipEventRecord");

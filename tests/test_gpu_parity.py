@@ -460,3 +460,33 @@ def test_fused_chunk_depth_and_autotune_leave_results_unchanged():
     assert np.array_equal(s.conserved(), outs[0])
     with pytest.raises(M.MHDError, match="at least 4"):
         M.set_chunk(s, 2)
+
+
+def test_ct_chunk_depth_and_autotune_leave_results_unchanged():
+    """The same for the fused CT step: the tuner writes only scratch fields and per-step
+    workspace, so cells and faces are untouched, and every depth gives the same bits."""
+    n = 48
+    U = P.config_state("c2_256", seed=4, n=n).astype(np.float32)
+
+    def ct_state(zc):
+        s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod", mhd_solver="ct")
+        s.set_conserved(U)
+        M.ct_faces_from_cells(s)
+        M.set_chunk(s, zc)
+        return s
+
+    a = ct_state(0)
+    before = (a.conserved(), [a.ct_slot(q) for q in (0, 1, 2, 18, 19, 20)])
+    zc, ms = M.autotune(a, reps=2)
+    assert zc in (4, 8, 12, 16, 24, 32, 48) and ms > 0
+    assert np.array_equal(a.conserved(), before[0])
+    for q, b in zip((0, 1, 2, 18, 19, 20), before[1]):
+        assert np.array_equal(a.ct_slot(q), b), q
+    b = ct_state(7)
+    for _ in range(3):
+        dt, _ = M.compute_dt(a, 0.3)
+        M.step(a, float(dt))
+        M.step(b, float(dt))
+    assert np.array_equal(a.conserved(), b.conserved())
+    for q in range(3):
+        assert np.array_equal(a.ct_slot(q), b.ct_slot(q))
