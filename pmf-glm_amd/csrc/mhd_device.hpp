@@ -318,7 +318,10 @@ __device__ __forceinline__ void riemann_n(T* L, T* R, T* f, const StepParams<T>&
     fpsi = T(0);
     ch = T(0);
   } else {
-    // glm_pair (glm_physics.jl:234-238)
+    // glm_pair (glm_physics.jl:234-238), uncontracted: the psi flux ch^2 bn* ends in a
+    // product, and a caller's flux difference F_lo - F_hi could otherwise fuse one face's
+    // product into an FMA, so that equal fluxes would not cancel
+#pragma clang fp contract(off)
     bns = T(0.5) * (L[5] + R[5]) - T(0.5) * (R[8] - L[8]) * P.ich;
     fbn = T(0.5) * (L[8] + R[8]) - T(0.5) * P.ch * (R[5] - L[5]);
     fpsi = P.ch2 * bns;

@@ -84,9 +84,14 @@ __device__ __forceinline__ bool surr_prepare(T* L, T* R, const StepParams<T>& P,
   L[4] = dmax(L[4], P.pfl);
   R[0] = dmax(R[0], P.smallr);
   R[4] = dmax(R[4], P.pfl);
-  T bns = T(0.5) * (L[5] + R[5]) - T(0.5) * (R[8] - L[8]) * P.ich;
-  fbn = T(0.5) * (L[8] + R[8]) - T(0.5) * P.ch * (R[5] - L[5]);
-  fpsi = P.ch2 * bns;
+  T bns;
+  {
+    // the GLM pair exactly as riemann_n evaluates it (no contraction)
+#pragma clang fp contract(off)
+    bns = T(0.5) * (L[5] + R[5]) - T(0.5) * (R[8] - L[8]) * P.ich;
+    fbn = T(0.5) * (L[8] + R[8]) - T(0.5) * P.ch * (R[5] - L[5]);
+    fpsi = P.ch2 * bns;
+  }
   L[5] = bns;
   R[5] = bns;
   return (P.llf_dmin > T(0) && dmin(L[0], R[0]) < P.llf_dmin) ||

@@ -91,3 +91,36 @@ def test_python_mirror_validates_like_the_reference():
 
 def test_version_string():
     assert b"gfx950" in _lib.lib().pmf_version()
+
+
+def _compile_capi_check(tmp_path):
+    """tests/capi_check.c as strict C99 against include/pmfmhd.h, linked to the library."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    root = ROOT
+    libdir = os.path.join(root, "pmf-glm_amd", "pmfmhd")
+    exe = str(tmp_path / "capi_check")
+    subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "capi_check.c"), "-L", libdir, "-lpmfmhd", f"-Wl,-rpath,{libdir}",
+                    "-o", exe], check=True)
+    return exe
+
+
+def test_header_is_c99_and_a_c_program_gets_codes_and_messages(tmp_path):
+    import subprocess
+    exe = _compile_capi_check(tmp_path)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "ok (no GPU calls)" in r.stdout
+
+
+@pytest.mark.gpu
+def test_a_c_program_steps_a_uniform_state_exactly(tmp_path):
+    import subprocess
+    exe = _compile_capi_check(tmp_path)
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "uniform state unchanged" in r.stdout
