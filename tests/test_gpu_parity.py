@@ -490,3 +490,40 @@ def test_ct_chunk_depth_and_autotune_leave_results_unchanged():
     assert np.array_equal(a.conserved(), b.conserved())
     for q in range(3):
         assert np.array_equal(a.ct_slot(q), b.ct_slot(q))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recon", ["plm", "plm_minmod", "ppm"])
+@pytest.mark.parametrize("riemann", ["hlld", "hll", "llf"])
+def test_uniform_state_is_a_fixed_point_bit_for_bit(dtype, recon, riemann):
+    """Every face of a uniform periodic state has the same two states, so every flux
+    difference must cancel exactly: one step on either integrator, a CT step, and the
+    drag step with a zero target velocity change nothing but the drag's own velocity
+    relaxation (here the state is at rest in the target frame, so nothing at all)."""
+    n = 16
+    rho, v, p, B = 1.3, (0.4, -0.2, 0.3), 0.8, (0.5, 0.2, -0.1)
+    e = p / (G - 1) + 0.5 * rho * sum(x * x for x in v) + 0.5 * sum(b * b for b in B)
+    U = np.zeros((9, n ** 3), dtype)
+    for k, val in enumerate((rho, rho * v[0], rho * v[1], rho * v[2], e, B[0], B[1], B[2], 0.0)):
+        U[k] = dtype(val)
+    for integrator in ("ref", "cube"):
+        s = M.allocate_state((n, n, n), 1.0 / n, dtype=dtype, recon=recon, riemann=riemann)
+        s.set_conserved(U)
+        dt, sm = M.compute_dt(s, 0.3)
+        M.step(s, float(dt), ch=float(sm), integrator=integrator)
+        assert np.array_equal(s.conserved(), U), integrator
+    c = M.allocate_state((n, n, n), 1.0 / n, dtype=dtype, recon=recon, riemann=riemann, mhd_solver="ct")
+    c.set_conserved(U)
+    M.ct_faces_from_cells(c)
+    Uc = c.conserved()
+    dt, _ = M.compute_dt(c, 0.3)
+    M.step(c, float(dt))
+    assert np.array_equal(c.conserved(), Uc)
+    r = U.copy()
+    r[1:4] = 0
+    r[4] = dtype(p / (G - 1) + 0.5 * sum(b * b for b in B))
+    d = M.allocate_state((n, n, n), 1.0 / n, dtype=dtype, recon=recon, riemann=riemann)
+    d.set_conserved(r)
+    dt, sm = M.compute_dt(d, 0.3)
+    M.step_drag(d, float(dt), float(sm), 0.99, 0.3, integrator="cube")
+    assert np.array_equal(d.conserved(), r)
