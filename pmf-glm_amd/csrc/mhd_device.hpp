@@ -126,17 +126,25 @@ template <int D, class T> __device__ __forceinline__ void dflux(const T* q, T* f
 // (glm_physics.jl:47-59 PLM MonCen / minmod, 188-200 PPM, 212-223).
 template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0, T qp, T& dL, T& dR) {
   if constexpr (RECON == RECON_PPM) {
+    // The unlimited parabola's edges are qL = q0 - b and qR = q0 + a. In these offsets the
+    // CW84 monotonization of glm_physics.jl:192-200 reads: flat (both 0) when a*b <= 0;
+    // otherwise diff > dq²/6 is |b| > 2|a| (then b := 2a), diff < -dq²/6 is |a| > 2|b|
+    // (then a := 2b). For a and b of one sign that is a := minmod(a, 2b),
+    // b := minmod(b, 2a), and minmod(x, y) = median(x, y, 0) covers the flat case too.
+    // Every branch is continuous in (a, b), so rounding cannot flip a result by more than
+    // rounding.
     T slope = (qp - qm) * T(0.25);
     T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
-    T qL = q0 - slope + curve, qR = q0 + slope + curve;
-    T dq = qR - qL;
-    T diff = (q0 - T(0.5) * (qL + qR)) * dq;
-    T lim = dq * dq * T(1.0 / 6.0);
-    bool flat = (qR - q0) * (q0 - qL) <= T(0);
-    T eL = flat ? q0 : (diff > lim ? T(3) * q0 - T(2) * qR : qL);
-    T eR = flat ? q0 : ((diff <= lim && diff < -lim) ? T(3) * q0 - T(2) * qL : qR);
-    dL = eL - q0;
-    dR = eR - q0;
+    T a = slope + curve, b = slope - curve;
+    if constexpr (sizeof(T) == 4) {
+      dR = __builtin_amdgcn_fmed3f(a, T(2) * b, T(0));
+      dL = -__builtin_amdgcn_fmed3f(b, T(2) * a, T(0));
+    } else {
+      const bool flat = a * b <= T(0);
+      const T a2 = T(2) * a, b2 = T(2) * b;
+      dR = flat ? T(0) : (dabs(a) > dabs(b2) ? b2 : a);
+      dL = flat ? T(0) : -(dabs(b) > dabs(a2) ? a2 : b);
+    }
   } else if constexpr (sizeof(T) == 4) {
     // minmod(a, b) is the median of (a, b, 0), and MonCen the median of (2 minmod(a, b),
     // (a+b)/2, 0): one v_med3_f32 each instead of the sign/abs/min/compare chain (equal

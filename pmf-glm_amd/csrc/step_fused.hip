@@ -556,12 +556,14 @@ __device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
   }
 }
 
-// Three waves per SIMD for the float variants whose registers fit 168 VGPRs without
-// spilling (three 38 KB workgroups then share a CU's LDS); the compiler's default
-// (two waves) for the rest.
+// Three waves per SIMD (168 VGPRs; three 38 KB workgroups then share a CU's LDS) for
+// every plain float variant. PLM fits without spilling; PPM, which keeps both edge
+// offsets of each direction live, spills 16-20 dwords and is still 12% faster than at
+// two waves (256^3 HLLD: 0.99 vs 1.13 ms). The compiler's default (two waves) for the
+// surrogate, source and float64 variants.
 template <class T, int RECON, int RS, bool SURR, bool SRC>
 constexpr bool fused_three_waves() {
-  return sizeof(T) == 4 && !SURR && !SRC && (RECON == RECON_PLM_MINMOD || (RECON == RECON_PLM && RS != RS_HLLD));
+  return sizeof(T) == 4 && !SURR && !SRC;
 }
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
@@ -932,8 +934,8 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 // update and workspace slots 3-14 (face mass and transverse-B fluxes of every cell's
 // lower faces, cell EMFs) that ct_update! reads.
 template <class T, int RECON, int RS>
-__global__ __launch_bounds__(FTHREADS) void k_step_fused_ct(Fields<const T> U, Fields<T> O, T* __restrict__ ctp,
-                                                            FusedGeom g, StepParams<T> P) {
+__device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, const Fields<T>& O, T* __restrict__ ctp,
+                                                   const FusedGeom& g, const StepParams<T>& P) {
   __shared__ FusedLds<T> lds;
   const int nwg = gridDim.x;
   int w = blockIdx.x;
@@ -944,13 +946,39 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused_ct(Fields<const T> U, F
                                                             nullptr, ctp);
 }
 
+template <class T, int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) void k_step_fused_ct(Fields<const T> U, Fields<T> O, T* __restrict__ ctp,
+                                                            FusedGeom g, StepParams<T> P) {
+  step_fused_ct_body<T, RECON, RS>(U, O, ctp, g, P);
+}
+
+// float: three waves per SIMD, as the plain step, where that does not spill (PPM would
+// otherwise take 174 VGPRs and run two waves: 1.81 vs 2.08 ms per 256^3 CT step with
+// HLLD). minmod-LLF spills at three waves and keeps two.
+template <int RECON, int RS>
+constexpr bool ct_three_waves() {
+  return !(RECON == RECON_PLM_MINMOD && RS == RS_LLF);
+}
+
+template <int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused_ct3(
+    Fields<const float> U, Fields<float> O, float* __restrict__ ctp, FusedGeom g, StepParams<float> P) {
+  step_fused_ct_body<float, RECON, RS>(U, O, ctp, g, P);
+}
+
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
                           hipStream_t st, int zc) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
-#define PMF_CT(R, S) hipLaunchKernelGGL((k_step_fused_ct<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P)
+#define PMF_CT(R, S)                                                                   \
+  do {                                                                                 \
+    if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>())                           \
+      hipLaunchKernelGGL((k_step_fused_ct3<R, S>), grd, blk, 0, st, U, O, ctp, g, P);  \
+    else                                                                               \
+      hipLaunchKernelGGL((k_step_fused_ct<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P); \
+  } while (0)
 #define PMF_CT_RS(R)                                \
   if (P.rsolve == RS_HLL) PMF_CT(R, RS_HLL);        \
   else if (P.rsolve == RS_LLF) PMF_CT(R, RS_LLF);   \
