@@ -56,7 +56,7 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
   if (ct && !ctp) throw Error("ct_update! needs a CT state");
   if (integrator == PMF_INTEGRATOR_CUBE) {
     if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
-      throw Error("step_cube!: the fused kernel needs an all-periodic grid whose nx is a multiple of 16 "
+      throw Error("step_cube!: the fused kernel needs nx a multiple of 16 and ny, nz >= 2 "
                   "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
                   ")");
     mark(s);
@@ -64,7 +64,8 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
         launch_step_fused<T>(in, out, s->G, make_params<T>(s, dt, ch, decay), s->red, s->dval, s->st, s->zc);
     mark(s);
     if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
-  } else if (ct && !s->ct_percell && fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann)) {
+  } else if (ct && !s->ct_percell && fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann) &&
+             all_periodic(s->G)) {
     // the same :ref algorithm with each face solved once (z-marching tiles)
     mark(s);
     launch_step_fused_ct<T>(in, out, ctp, s->G, make_params<T>(s, dt, 0.0, 1.0), s->st, s->zc);
@@ -169,7 +170,7 @@ void step_fields_src(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, do
   }
   if (cube) {
     if (!fused_supported(s->G, s->p.precision, P.recon, P.rsolve))
-      throw Error(std::string(what) + ": the fused kernel needs an all-periodic grid whose nx is a multiple of 16 "
+      throw Error(std::string(what) + ": the fused kernel needs nx a multiple of 16 and ny, nz >= 2 "
                                       "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" +
                   std::to_string(s->p.nz) + ")");
     mark(s);

@@ -26,6 +26,7 @@ template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
                        double* dsmax, hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
+bool all_periodic(const Grid& G);
 // The fused CT step (constrained-transport states, the :ref algorithm with each face
 // solved once): the cell update into O and workspace slots 3-14 of ctp; ct_update! then
 // finishes the step. Returns false when the grid shape is not supported.
@@ -46,6 +47,7 @@ constexpr int kMinChunk = 4;  // smallest z chunk the library accepts
 // then y; the last tile row may be ragged) marched through z chunks of zc planes.
 struct FusedGeom {
   int32_t nx, ny, nz, ntx, nty, nchunks, zc;
+  int32_t bc[3];  // per-axis boundary codes (BC_PERIODIC, BC_OUTFLOW, BC_REFLECT)
   int64_t plane, nc;
 };
 constexpr int kFusedTX = 16, kFusedTY = 12;

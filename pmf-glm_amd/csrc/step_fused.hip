@@ -45,6 +45,25 @@ __device__ __forceinline__ int wrapi(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// Source index of a possibly out-of-range logical index along one axis (boundary.jl:17-31,
+// 0-based): periodic wrap, outflow clamp, reflecting mirror about the boundary face with
+// flip set (the normal velocity and field change sign). The mirror is clamped into the
+// axis as well: only the rows of a ragged last tile row, which are computed but never
+// stored, reach past the two ghost layers.
+__device__ __forceinline__ int src_index(int i, int n, int bc, bool& flip) {
+  flip = false;
+  if (bc == BC_PERIODIC) return wrapi(i, n);
+  if (i >= 0 && i < n) return i;
+  int r;
+  if (bc == BC_OUTFLOW) {
+    r = i < 0 ? 0 : n - 1;
+  } else {
+    flip = true;
+    r = i < 0 ? -1 - i : 2 * n - 1 - i;
+  }
+  return min(max(r, 0), n - 1);
+}
+
 // Address of cell idx of one field. NARROW: the byte offset fits 32 bits (fields below
 // 4 GiB), so every access is the field's scalar base plus one 32-bit vector offset shared
 // by the nine fields (global_load/store with saddr) instead of a 64-bit address each.
@@ -129,13 +148,15 @@ struct FusedLds {
 // learned flux (wave-level MFMA) or by HLLD; sW holds the network's LDS image
 // (surrogate.hpp), sB its biases (global memory), and smooth_count counts the faces that
 // used it (each face once, as the lower face of its cell).
-template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false,
+          bool BCG = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                         const StepParams<T>& P, int w, FusedLds<T, SRC>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
                                         SurrQueue<FQCAP>* Qp = nullptr, T* __restrict__ ctp = nullptr) {
   static_assert(!(CT && (SURR || SRC)), "the fused CT step has no surrogate or source terms");
+  static_assert(!(BCG && (CT || SURR)), "the CT and surrogate steps run on periodic grids");
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -148,10 +169,44 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   const bool has_col = column_of(t, ix, iy);
   const bool owned = t < NOWN;
   const bool stores = owned && y0 + iy < g.ny;  // ragged last tile row: compute, do not store
-  const int64_t col = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
+  // the column's periodic image: its source cells on periodic grids, and the index of the
+  // radiation correction buffer on every grid (integrator_ref.jl:50-53)
+  const int64_t pcol = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
   int ex = 0, ey = 0;
   const bool has_extra = extra_of(t, ex, ey);
-  const int64_t ecol = has_extra ? (int64_t)wrapi(y0 + ey, g.ny) * g.nx + wrapi(x0 + ex, g.nx) : 0;
+  int64_t col, ecol = 0;
+  int flips = 0;  // BCG: bit 0/1 = x/y mirror of the column, bit 2/3 = of the extra cell
+  if constexpr (BCG) {
+    // source cells under outflow / reflecting boundaries (boundary.jl:17-31)
+    bool fa, fb;
+    col = (int64_t)src_index(y0 + iy, g.ny, g.bc[1], fb) * g.nx + src_index(x0 + ix, g.nx, g.bc[0], fa);
+    flips = (fa ? 1 : 0) | (fb ? 2 : 0);
+    if (has_extra) {
+      ecol = (int64_t)src_index(y0 + ey, g.ny, g.bc[1], fb) * g.nx + src_index(x0 + ex, g.nx, g.bc[0], fa);
+      flips |= (fa ? 4 : 0) | (fb ? 8 : 0);
+    }
+  } else {
+    col = pcol;
+    if (has_extra) ecol = (int64_t)wrapi(y0 + ey, g.ny) * g.nx + wrapi(x0 + ex, g.nx);
+  }
+  // conserved state of logical plane z of a column (sh = 0) or of the extra cell (sh = 2);
+  // BCG: the source plane and the sign flips of the reflected normal components
+  // (boundary.jl:34-41)
+  auto load_plane = [&](int z, int64_t c2, int sh, T* c) {
+    if constexpr (BCG) {
+      bool fz = false;
+      int kz = z;
+      if (z < 0 || z >= g.nz) kz = src_index(z, g.nz, g.bc[2], fz);
+      load_cons<NARROW>(U, (int64_t)kz * g.plane + c2, c);
+      const int f = (flips >> sh) & 3;
+      if (f & 1) { c[1] = -c[1]; c[5] = -c[5]; }
+      if (f & 2) { c[2] = -c[2]; c[6] = -c[6]; }
+      if (fz) { c[3] = -c[3]; c[7] = -c[7]; }
+    } else {
+      (void)sh;
+      load_cons<NARROW>(U, (int64_t)wrapi(z, g.nz) * g.plane + c2, c);
+    }
+  };
   const bool xsolver = has_col && iy >= 0 && iy < TY && ix >= 0;  // owned + right ring column
   const bool ysolver = has_col && ix >= 0 && ix < TX && iy >= 0;  // owned + top ring row
   const bool xleft = has_col && ix < TX && iy >= 0 && iy < TY;     // owned + left ring column
@@ -167,11 +222,11 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   {
     T c[9], q[9];
     if (has_col) {
-      load_cons<NARROW>(U, (int64_t)wrapi(z0 - 2, g.nz) * g.plane + col, c);
+      load_plane(z0 - 2, col, 0, c);
       cons2prim(c, q, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) S.sP[(z0 - 2) & 1][v * FNP + me] = q[v];
-      load_cons<NARROW>(U, (int64_t)wrapi(z0 - 1, g.nz) * g.plane + col, c);
+      load_plane(z0 - 1, col, 0, c);
       cons2prim(c, q, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) S.sP[(z0 - 1) & 1][v * FNP + me] = q[v];
@@ -233,9 +288,9 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     T* __restrict__ nxt = S.sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
     // (1) plane m+1 of the own column and of the extra cell (in flight during the barrier)
     T Pn[9], Xc[9];
-    if (has_col) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + col, Pn);
+    if (has_col) load_plane(m + 1, col, 0, Pn);
     const bool extra_now = has_extra && m + 1 < z0 + zc;
-    if (extra_now) load_cons<NARROW>(U, (int64_t)wrapi(m + 1, g.nz) * g.plane + ecol, Xc);
+    if (extra_now) load_plane(m + 1, ecol, 2, Xc);
     if constexpr (SURR) {
       if (t == 0) Qp->n = 0u;  // the previous plane's queue was last read before its final barrier
     }
@@ -268,7 +323,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if constexpr (SRC) {
         // masked / global fallback: flagged cells are predicted with zero slopes
         // (integrator_cube.jl:330-335)
-        pcell = (int64_t)wrapi(m, g.nz) * g.plane + col;
+        pcell = (int64_t)wrapi(m, g.nz) * g.plane + pcol;
         const bool mk = src.check == CHECK_GLOBAL ||
                         (src.check == CHECK_MASKED && src.corr[4 * src.stride + pcell] > T(0));
         if (mk) {
@@ -568,7 +623,7 @@ constexpr bool fused_three_waves() {
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
 // global memory (SURR only) and nsmooth the count of network-solved faces.
-template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW, bool BCG = false>
 __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                                                 const StepParams<T>& P, double* __restrict__ red,
                                                 const unsigned char* __restrict__ simg, float thr,
@@ -593,7 +648,8 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   // the biases (the image's float tail) stay in global memory: with them out of LDS the
   // surrogate step's 81.6 KB lets two workgroups share a CU
   const float* sB = SURR ? reinterpret_cast<const float*>(simg + SIMG_NW * 2) : nullptr;
-  const T smax = fused_item<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src, q);
+  const T smax =
+      fused_item<T, RECON, RS, SURR, SRC, NARROW, false, BCG>(U, O, g, P, w, lds, sW, sB, thr, smooth_count, src, q);
   // fold the workgroup's CFL maximum (timestep.jl:8-12 on the new state)
   block_store_max(smax, sMax, red + blockIdx.x);
   if constexpr (SURR) {
@@ -603,19 +659,19 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   }
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW, bool BCG>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
                                                          StepParams<T> P, double* __restrict__ red,
                                                          const unsigned char* __restrict__ simg, float thr,
                                                          unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
-  step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
+  step_fused_body<T, RECON, RS, SURR, SRC, NARROW, BCG>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW>
+template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW, bool BCG>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused3(
     Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
     const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
-  step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
+  step_fused_body<T, RECON, RS, SURR, SRC, NARROW, BCG>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
 // The surrogate-assisted step: two waves per SIMD (about 250 VGPRs with the network in its
@@ -628,19 +684,19 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2))) v
   step_fused_body<T, RECON, RS, SURR, SRC, NARROW>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
-template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false>
+template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool BCG = false>
 void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                           const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
                           unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{}) {
   if constexpr (fused_three_waves<T, RECON, RS, SURR, SRC>())
-    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
-                       simg, thr, nsmooth, src);
+    hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC, NARROW, BCG>), grd, dim3(FTHREADS), 0, st, U, O, g, P,
+                       red, simg, thr, nsmooth, src);
   else if constexpr (SURR)
     hipLaunchKernelGGL((k_step_fused2<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
                        simg, thr, nsmooth, src);
   else
-    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
-                       simg, thr, nsmooth, src);
+    hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC, NARROW, BCG>), grd, dim3(FTHREADS), 0, st, U, O, g, P,
+                       red, simg, thr, nsmooth, src);
 }
 
 // The plain float step with 32-bit offsets (double steps never take this path).
@@ -676,6 +732,7 @@ static FusedGeom fused_geom(const Grid& G, int zc) {
   g.nchunks = (G.nz + zc - 1) / zc;
   g.plane = (int64_t)G.nx * G.ny;
   g.nc = G.nc;
+  for (int d = 0; d < 3; ++d) g.bc[d] = G.bc[d];
   return g;
 }
 
@@ -683,8 +740,11 @@ bool fused_supported(const Grid& G, int precision, int recon, int rsolve) {
   (void)precision;
   (void)recon;
   (void)rsolve;
-  return G.bc[0] == BC_PERIODIC && G.bc[1] == BC_PERIODIC && G.bc[2] == BC_PERIODIC && G.nx % TX == 0 &&
-         G.ny >= 2 && G.nz >= 2;
+  return G.nx >= TX && G.nx % TX == 0 && G.ny >= 2 && G.nz >= 2;
+}
+
+bool all_periodic(const Grid& G) {
+  return G.bc[0] == BC_PERIODIC && G.bc[1] == BC_PERIODIC && G.bc[2] == BC_PERIODIC;
 }
 
 // z-chunk length: long enough that the two extra prediction planes per chunk stay
@@ -712,9 +772,13 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
   const bool narrow = sizeof(T) == 4 && narrow_fields<T>(G.nc);
-#define PMF_FUSED(R, S)                                                                                          \
-  if (src) launch_fused_variant<T, R, S, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src);    \
-  else if (narrow) launch_plain_narrow<T, R, S>(grd, st, U, O, g, P, red);                                        \
+  const bool gen = !all_periodic(G);  // outflow / reflecting axes: the BCG variants
+#define PMF_FUSED(R, S)                                                                                             \
+  if (gen && src)                                                                                                  \
+    launch_fused_variant<T, R, S, false, true, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src); \
+  else if (gen) launch_fused_variant<T, R, S, false, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr); \
+  else if (src) launch_fused_variant<T, R, S, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src);   \
+  else if (narrow) launch_plain_narrow<T, R, S>(grd, st, U, O, g, P, red);                                          \
   else launch_fused_variant<T, R, S, false, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
@@ -736,7 +800,7 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
 bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
                             double* red, double* dsmax, const unsigned char* simg, float thr,
                             unsigned long long* nsmooth, hipStream_t st) {
-  if (!fused_supported(G, 0, P.recon, P.rsolve) || P.rsolve != RS_HLLD) return false;
+  if (!fused_supported(G, 0, P.recon, P.rsolve) || !all_periodic(G) || P.rsolve != RS_HLLD) return false;
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);
@@ -890,6 +954,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) v
 FusedGeom ens_geom(int n, int zc) {
   Grid G;
   G.nx = G.ny = G.nz = n;
+  G.bc[0] = G.bc[1] = G.bc[2] = BC_PERIODIC;
   G.nc = (int64_t)n * n * n;
   return fused_geom(G, zc);
 }
@@ -969,7 +1034,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) v
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
                           hipStream_t st, int zc) {
-  if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
+  if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve) || !all_periodic(G)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
 #define PMF_CT(R, S)                                                                   \

@@ -270,6 +270,30 @@ def test_ref_drag_with_a_moving_target_and_boundaries_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bcs", [("outflow", "reflecting", "periodic"), ("reflecting", "reflecting", "outflow")])
+def test_cube_coupled_steps_on_bounded_grids_match_oracle(bcs):
+    """step_cube_drag! / step_cube_radiation! on outflow / reflecting grids
+    (integrator_cube.jl:478-540 with per = false): ghost cells from the boundary map, the
+    radiation corrections and masks at the periodic image of each cell
+    (integrator_ref.jl:50-53)."""
+    import pmfmhd as M
+    n = 16
+    U = smooth_state(n, seed=12)
+    s, o = _pair(n, U, np.float64, "ppm", "hlld", bcs=bcs)
+    dt, ch = o.compute_dt(0.3)
+    M.step_drag(s, dt, ch, 0.95, 0.3, integrator="cube")
+    o.step_sources(dt, ch, 0.95, drag_impulse=0.3)
+    assert _rel(s.conserved(), o.U) < 1e-12
+    c = _corr(n, o.U, seed=5)
+    c[4] = (np.random.default_rng(6).random(n ** 3) < 0.1).astype(np.float64)
+    buf = M.correction_buffer(s, c)
+    M.step_radiation(s, dt, ch, 0.9, 0.2, buf, fallback=True, track_density=False, integrator="cube")
+    o.step_sources(dt, ch, 0.9, drag_impulse=0.2, correction=c, track_density=False, track_magnetic=True,
+                   track_energy=True, check=2)
+    assert _rel(s.conserved(), o.U) < 1e-12
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("integ", ["ref", "cube"])
 def test_radiation_step_with_tracking_matches_oracle_float64(integ):
     import pmfmhd as M
