@@ -56,7 +56,7 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
   if (ct && !ctp) throw Error("ct_update! needs a CT state");
   if (integrator == PMF_INTEGRATOR_CUBE) {
     if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
-      throw Error("step_cube!: the fused kernel needs nx a multiple of 16 and ny, nz >= 2 "
+      throw Error("step_cube!: the fused kernel needs every axis at least one cell and reflecting axes two "
                   "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) +
                   ")");
     mark(s);
@@ -170,7 +170,7 @@ void step_fields_src(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, do
   }
   if (cube) {
     if (!fused_supported(s->G, s->p.precision, P.recon, P.rsolve))
-      throw Error(std::string(what) + ": the fused kernel needs nx a multiple of 16 and ny, nz >= 2 "
+      throw Error(std::string(what) + ": the fused kernel needs every axis at least one cell and reflecting axes two "
                                       "(got " + std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" +
                   std::to_string(s->p.nz) + ")");
     mark(s);

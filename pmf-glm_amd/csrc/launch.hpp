@@ -27,6 +27,7 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
                        double* dsmax, hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
 bool all_periodic(const Grid& G);
+bool surrogate_grid(const Grid& G);
 // The fused CT step (constrained-transport states, the :ref algorithm with each face
 // solved once): the cell update into O and workspace slots 3-14 of ctp; ct_update! then
 // finishes the step. Returns false when the grid shape is not supported.

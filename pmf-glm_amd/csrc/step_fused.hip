@@ -168,7 +168,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   int ix, iy;
   const bool has_col = column_of(t, ix, iy);
   const bool owned = t < NOWN;
-  const bool stores = owned && y0 + iy < g.ny;  // ragged last tile row: compute, do not store
+  // ragged last tile column / row: computed (their cells feed no stored cell), not stored
+  const bool stores = owned && x0 + ix < g.nx && y0 + iy < g.ny;
   // the column's periodic image: its source cells on periodic grids, and the index of the
   // radiation correction buffer on every grid (integrator_ref.jl:50-53)
   const int64_t pcol = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
@@ -726,7 +727,7 @@ static FusedGeom fused_geom(const Grid& G, int zc) {
   g.nx = G.nx;
   g.ny = G.ny;
   g.nz = G.nz;
-  g.ntx = G.nx / TX;
+  g.ntx = (G.nx + TX - 1) / TX;
   g.nty = (G.ny + TY - 1) / TY;
   g.zc = zc;
   g.nchunks = (G.nz + zc - 1) / zc;
@@ -736,21 +737,30 @@ static FusedGeom fused_geom(const Grid& G, int zc) {
   return g;
 }
 
+// Any grid: ragged last tiles and chunks are computed and not stored, and out-of-range
+// columns and planes come from the boundary map. A reflecting axis needs two cells (its
+// second ghost layer mirrors onto the second cell, boundary.jl:24-28).
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve) {
   (void)precision;
   (void)recon;
   (void)rsolve;
-  return G.nx >= TX && G.nx % TX == 0 && G.ny >= 2 && G.nz >= 2;
+  const int32_t n[3] = {G.nx, G.ny, G.nz};
+  for (int d = 0; d < 3; ++d)
+    if (n[d] < 1 || (G.bc[d] == BC_REFLECT && n[d] < 2)) return false;
+  return true;
 }
 
 bool all_periodic(const Grid& G) {
   return G.bc[0] == BC_PERIODIC && G.bc[1] == BC_PERIODIC && G.bc[2] == BC_PERIODIC;
 }
 
+// The surrogate step's grids: periodic, whole x tiles, ny and nz of at least two.
+bool surrogate_grid(const Grid& G) { return all_periodic(G) && G.nx % TX == 0 && G.ny >= 2 && G.nz >= 2; }
+
 // z-chunk length: long enough that the two extra prediction planes per chunk stay
 // small, short enough to give a few workgroups per CU.
 static int pick_zc(const Grid& G) {
-  int64_t tiles = (int64_t)(G.nx / TX) * ((G.ny + TY - 1) / TY);
+  int64_t tiles = (int64_t)((G.nx + TX - 1) / TX) * ((G.ny + TY - 1) / TY);
   int zc = 32;
   while (zc > 4 && tiles * ((G.nz + zc - 1) / zc) < 2048) zc >>= 1;
   return zc < G.nz ? zc : G.nz;
@@ -759,7 +769,6 @@ static int pick_zc(const Grid& G) {
 static int chunk_of(const Grid& G, int zc) { return zc > 0 ? (zc < G.nz ? zc : G.nz) : pick_zc(G); }
 
 int64_t fused_blocks(const Grid& G, int zc) {
-  if (G.nx % TX) return 0;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   return (int64_t)g.ntx * g.nty * g.nchunks;
 }
@@ -800,7 +809,7 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
 bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
                             double* red, double* dsmax, const unsigned char* simg, float thr,
                             unsigned long long* nsmooth, hipStream_t st) {
-  if (!fused_supported(G, 0, P.recon, P.rsolve) || !all_periodic(G) || P.rsolve != RS_HLLD) return false;
+  if (!surrogate_grid(G) || P.rsolve != RS_HLLD) return false;
   FusedGeom g = fused_geom(G, pick_zc(G));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
   dim3 blk(FTHREADS), grd(nb);

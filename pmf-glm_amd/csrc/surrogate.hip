@@ -613,7 +613,7 @@ int pmf_step_surrogate(pmf_state* s, pmf_surrogate* n, double dt, double ch, dou
     if (s->p.solver != PMF_SOLVER_GLM) throw Error("the flux surrogate replaces the GLM-HLLD flux (GLM states only)");
     if (s->p.riemann != PMF_RIEMANN_HLLD) throw Error("the flux surrogate stands in for HLLD (riemann=:hlld)");
     if (!(ch > 0.0)) throw Error("step! needs the GLM cleaning speed `ch` (GLM states)");
-    if (!fused_supported(s->G, 0, s->p.recon, s->p.riemann) || !all_periodic(s->G))
+    if (!surrogate_grid(s->G))
       throw Error("the surrogate step needs an all-periodic grid whose nx is a multiple of 16");
     const double decay = decay_of(s, dt, ch, glm_cr);
     hipLaunchKernelGGL(k_surr_image, dim3((SIMG_NW + 255) / 256), dim3(256), 0, s->st, n->w, n->img);

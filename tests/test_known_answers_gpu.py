@@ -1,9 +1,8 @@
 """The reference's recorded known answers (validation-cpu.log, transcribed in
 tests/golden/known_answers.json) reproduced by the HIP library: Alfvén-wave convergence
 (PLM, PPM), low-beta Alfvén, Brio-Wu, Orszag-Tang and field-loop advection
-(test/inherited/runtests.jl:6-122,486-509), on the per-cell integrator and, where the
-grid allows it (periodic, nx a multiple of 16, nz >= 2), on the fused one. Float32,
-the reference's own precision for these records."""
+(test/inherited/runtests.jl:6-122,486-509), on the per-cell and on the fused integrator.
+Float32, the reference's own precision for these records."""
 import json
 import os
 
@@ -54,14 +53,15 @@ def test_low_beta_alfven(integrator):
     assert de < 2e-4
 
 
-def test_brio_wu():
-    """runtests.jl:65-91; validation-cpu.log:22 (outflow x boundaries: per-cell kernel)."""
+@pytest.mark.parametrize("integrator", ["ref", "cube"])
+def test_brio_wu(integrator):
+    """runtests.jl:65-91; validation-cpu.log:22 (outflow x boundaries)."""
     def density(N):
         s = M.allocate_state((N, 4, 4), 1.0 / N, dtype=np.float32, gamma=2.0,
                              bcs=("outflow", "periodic", "periodic"))
         s.set_conserved(P.brio_wu((N, 4, 4), gamma=2.0))
         t0 = M.conserved_totals(s)
-        M.evolve(s, 0.1, cfl=0.4)
+        M.evolve(s, 0.1, cfl=0.4, integrator=integrator)
         t1 = M.conserved_totals(s)
         return s.download(0)[:N].astype(np.float64), abs(t1["mass"] - t0["mass"]) / t0["mass"]
     r2, _ = density(256)
@@ -74,13 +74,14 @@ def test_brio_wu():
     assert dm4 < 1e-4
 
 
-def test_orszag_tang():
-    """runtests.jl:94-122; validation-cpu.log:25 (128^2 x 1: per-cell kernel)."""
+@pytest.mark.parametrize("integrator", ["ref", "cube"])
+def test_orszag_tang(integrator):
+    """runtests.jl:94-122; validation-cpu.log:25 (128^2 x 1)."""
     N = 128
     s = M.allocate_state((N, N, 1), 1.0 / N, dtype=np.float32)
     s.set_conserved(P.orszag_tang((N, N, 1), 1.0 / N))
     t0 = M.conserved_totals(s)
-    t, n = M.evolve(s, 0.5, cfl=0.4)
+    t, n = M.evolve(s, 0.5, cfl=0.4, integrator=integrator)
     t1 = M.conserved_totals(s)
     h = np.stack(s.fields_to_host()).astype(np.float64)
     brms = np.sqrt(np.mean(h[5] ** 2 + h[6] ** 2 + h[7] ** 2))
