@@ -84,7 +84,9 @@ int pmf_compute_dt(pmf_state* s, double cfl, double* dt, double* smax);
  * MUSCL-Hancock step of size dt with GLM speed ch and psi factor decay, written to
  * scratch and swapped in; CT states also run ct_update! (ct.jl:158-173) and ignore ch
  * and decay. integrator: PMF_INTEGRATOR_REF (per-cell recompute) or
- * PMF_INTEGRATOR_CUBE (fused tiled kernel; GLM only). */
+ * PMF_INTEGRATOR_CUBE (fused tiled kernel; GLM only; any grid shape and boundary codes,
+ * where the reference's needs a cubic N % 4 == 0 grid; a reflecting axis needs two
+ * cells). */
 int pmf_step_raw(pmf_state* s, double dt, double ch, double decay, int32_t integrator);
 /* step! (driver.jl:20-37): decay = exp(-glm_cr*ch*dt/dx) in the state's precision. */
 int pmf_step(pmf_state* s, double dt, double ch, double glm_cr, int32_t integrator);
@@ -131,9 +133,9 @@ int pmf_set_chunk(pmf_state* s, int32_t zc);
  * CT, the per-step workspace slots) and keeps the fastest; returns it and its time per
  * step. */
 int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
-/* Kernel of the CT (:ref) step. By default a constrained-transport state whose grid the
- * fused kernel supports (nx a multiple of 16) steps on the fused z-marching kernel, which
- * solves every face once with the :ref algorithm (integrator_ref.jl:83-139); per_cell != 0
+/* Kernel of the CT (:ref) step. By default a constrained-transport state steps on the
+ * fused z-marching kernel, which solves every face once with the :ref algorithm
+ * (integrator_ref.jl:83-139); per_cell != 0
  * selects the per-cell kernel of integrator_ref.jl:189-220 instead. No counterpart in the
  * reference (one kernel there); results agree to rounding. */
 int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell);
