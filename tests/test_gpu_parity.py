@@ -111,7 +111,7 @@ def test_ref_step_boundaries_match_oracle(bcs):
 
 
 def test_config1_one_step_float32_vs_float32_oracle():
-    """North-star gate: float32 GPU vs the float32 CPU run <= 1e-5 after one step."""
+    """Acceptance bar: float32 GPU vs the float32 CPU run <= 1e-5 after one step."""
     n = 32
     U = P.config_state("c1_32", seed=1)
     g, o = pair((n, n, n), U, dtype=np.float32, recon="plm_minmod")

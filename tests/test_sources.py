@@ -307,7 +307,7 @@ def test_radiation_step_with_tracking_matches_oracle_float64(integ):
         M.step_radiation(s, dt, ch, 0.9, 0.4, buf, integrator=integ)
         o.step_sources(dt, ch, 0.9, drag_impulse=0.4, correction=c, track_density=True, track_magnetic=True,
                        track_energy=True)
-        c[0] = o.U[8]  # the driver reloads psi into slot 0 before each step
+        c[0] = o.U[8]  # the radiation loop reloads psi into slot 0 before each step (runner.jl)
         M.load_psi(s, buf)
     assert _rel(s.conserved(), o.U) < 1e-12
     got = buf.download(np.float64, (9, n ** 3))
