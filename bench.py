@@ -250,6 +250,51 @@ def ct_leg(M, P, n, steps):
     return out
 
 
+def refbench_leg(M, P, n, iters, solver):
+    """The reference's own benchmark (scripts/benchmark.jl:41-72): float32, PPM, HLLD,
+    the init_turb_field! state (problems.jl:9-29), one dt and smax from compute_dt at CFL
+    0.2, then `iters` blocking steps with that dt (ch = smax; CT ignores it) on the GPU
+    default integrator (GLM: cube; CT: ref), N^3 iters / wall time. The recorded figures
+    are the reference's only GPU throughput numbers (Apple M5 Max, Metal)."""
+    import numpy as np
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="ppm", riemann="hlld", mhd_solver=solver)
+    U = P.turb_field((n, n, n), 1.0 / n)
+
+    def init():
+        s.set_conserved(U)
+        if solver == "ct":
+            M.ct_faces_from_cells(s)
+
+    integ = "ref" if solver == "ct" else "cube"
+    init()
+    dt, sm = M.compute_dt(s, 0.2)
+
+    def advance():
+        if solver == "ct":
+            M.step(s, float(dt), integrator=integ)
+        else:
+            M.step(s, float(dt), ch=float(sm), integrator=integ)
+
+    advance()
+    init()  # the warm-up must not advance the measured state (benchmark.jl:50)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        advance()
+    s.synchronize()
+    el = time.perf_counter() - t0
+    h = s.fields_to_host()
+    ref = {"glm": (111.05188196743987, "docs/data/benchmark-metal.toml:10 (N=128, 100 iterations)"),
+           "ct": (133.1, "docs/performance.qmd:52-57 (CT ref, N=96, 60 iterations, mean of 131.9 and 134.3)")}
+    mc = n ** 3 * iters / el / 1e6
+    return {"workload": f"scripts/benchmark.jl --mhd-solver={solver}: float32 PPM HLLD {n}^3, init_turb_field!, "
+                        f"fixed CFL-0.2 dt, {iters} steps, integrator {integ}",
+            "mcells_per_second": mc, "reference_mcells_per_second": ref[solver][0],
+            "reference_source": ref[solver][1], "reference_hardware": "Apple M5 Max GPU (Metal)",
+            "ratio_to_reference": mc / ref[solver][0],
+            "finite": bool(all(np.isfinite(a).all() for a in h)), "positive_density": bool(h[0].min() > 0)}
+
+
 def traffic_from_profile(n, integ):
     """HBM bytes per fused launch from the committed FETCH_SIZE / WRITE_SIZE summary."""
     path = os.path.join(ROOT, "profiles", "k_step_fused_256cubed_traffic.md")
@@ -281,6 +326,8 @@ def main():
     ap.add_argument("--fp64-oracle-steps", type=int, default=10,
                     help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~4 s per step)")
     ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
+    ap.add_argument("--refbench", action=argparse.BooleanOptionalAction, default=True,
+                    help="the reference's own benchmark workload (scripts/benchmark.jl), GLM at N=128 and CT at N=96")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
                          "size that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -383,6 +430,10 @@ def main():
     if args.ct_steps > 0:
         log(f"bench: CT leg ({args.ct_steps} steps)")
         ct = ct_leg(M, P, n, args.ct_steps)
+    refb = None
+    if args.refbench:
+        log("bench: the reference's benchmark workload")
+        refb = {"glm_n128": refbench_leg(M, P, 128, 100, "glm"), "ct_n96": refbench_leg(M, P, 96, 60, "ct")}
 
     kernel = "k_step_fused" if integ == "cube" else "k_step_ref"
     out = {
@@ -412,6 +463,7 @@ def main():
         "c3_512": c3,
         "ct": ct,
         "fp64_oracle_check": orc,
+        "reference_benchmark": refb,
     }
     print(json.dumps(out), flush=True)
 
