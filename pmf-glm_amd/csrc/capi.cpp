@@ -77,7 +77,9 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
   }
   hip_check(hipGetLastError(), "step launch");
   if (ct) {
-    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), false, s->st);
+    // ct_update! with the new state's CFL speed folded into dval[0] by its last pass
+    const StepParams<T> Pw = make_params<T>(s, 0, 0, 1);
+    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), false, s->st, &Pw, s->red, s->dval);
     hip_check(hipGetLastError(), "ct_update launch");
   }
 }
@@ -185,7 +187,9 @@ void step_fields_src(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, do
   hip_check(hipGetLastError(), "step launch");
   if (ct) {
     // compensated faces exactly when GLM would compensate cell B (integrator_ref.jl:255-256)
-    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), a.radiation && (a.flags & PMF_SRC_TRACK_MAGNETIC), s->st);
+    const StepParams<T> Pw = make_params<T>(s, 0, 0, 1);
+    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), a.radiation && (a.flags & PMF_SRC_TRACK_MAGNETIC), s->st,
+                        &Pw, s->red, s->dval);
     hip_check(hipGetLastError(), "ct_update launch");
   }
 }
@@ -199,7 +203,8 @@ void do_step_src(pmf_state* s, double dt, double ch, double decay, int integrato
     step_fields_src<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), (float*)s->ct, dt, ch,
                            decay, integrator, a);
   std::swap(s->U, s->S);
-  s->smax_valid = integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM;
+  // the fused GLM epilogue or the CT cell-field pass reduced the new state's CFL speed
+  s->smax_valid = (integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM) || s->p.solver == PMF_SOLVER_CT;
 }
 
 // step_ref! / step_cube! on the state's own fields: into scratch, then swap.
@@ -212,8 +217,9 @@ void do_step(pmf_state* s, double dt, double ch, double decay, int integrator) {
     step_fields<float>(s, cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc), (float*)s->ct, dt, ch,
                        decay, integrator);
   std::swap(s->U, s->S);
-  // the fused epilogue reduced the new state's CFL speed into dval[0]
-  s->smax_valid = integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM;
+  // the fused GLM epilogue or the CT cell-field pass reduced the new state's CFL speed
+  // into dval[0]
+  s->smax_valid = (integrator == PMF_INTEGRATOR_CUBE && s->p.solver == PMF_SOLVER_GLM) || s->p.solver == PMF_SOLVER_CT;
 }
 
 }  // namespace
@@ -703,6 +709,7 @@ int pmf_step_into(pmf_state* s, const void* const* U, void* const* O, void* ct, 
   return guard([&] {
     check_state(s);
     if (!U || !O) throw Error("null field array");
+    s->smax_valid = false;  // dval[0] now describes the caller's output arrays
     if (s->p.precision == PMF_F64)
       step_fields<double>(s, cfields_from<double>(U), fields_from<double>(O), (double*)ct, dt, ch, decay, integrator);
     else
@@ -714,6 +721,7 @@ int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax) {
   return guard([&] {
     check_state(s);
     if (!U || !smax) throw Error("null argument");
+    s->smax_valid = false;  // dval[0] now describes the caller's arrays
     if (s->p.precision == PMF_F64)
       launch_wavespeed<double>(cfields_from<double>(U), s->G, make_params<double>(s, 0, 0, 1), s->red, s->dval, s->st);
     else

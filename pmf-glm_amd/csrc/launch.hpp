@@ -91,7 +91,11 @@ void launch_wavespeed(Fields<const T> U, const Grid& G, const StepParams<T>& P, 
 template <class T>
 void launch_divb(Fields<const T> U, const Grid& G, T inv2dx, double* red, double* out3, hipStream_t st);
 template <class T> void launch_totals(Fields<const T> U, const Grid& G, double* red, double* out8, hipStream_t st);
-template <class T> void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st);
+// ct_update! (ct.jl:158-173) on U. With P, red and dsmax: the cell-field pass also folds
+// the CFL speed of the finished state into *dsmax (device), as the fused GLM step does.
+template <class T>
+void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st,
+                      const StepParams<T>* P = nullptr, double* red = nullptr, double* dsmax = nullptr);
 template <class T> void launch_divb_face(const T* ctp, const Grid& G, T invdx, double* red, double* out, hipStream_t st);
 template <class T>
 void launch_ct_install(Fields<T> U, T* ctp, const Grid& G, hipStream_t st);

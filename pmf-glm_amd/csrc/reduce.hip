@@ -282,12 +282,47 @@ template <class T> __global__ __launch_bounds__(kBlk) void k_ct_bcc(Fields<T> U,
   U.v[7][c] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
 }
 
-template <class T> void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st) {
+// ct.jl:132-143 and timestep.jl:8-12 in one pass: the cell field from the faces, then
+// the CFL speed of the finished cell (the step's final state), folded per block.
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_ct_bcc_cfl(Fields<T> U, const T* __restrict__ ctp, Grid G, StepParams<T> P,
+                                                     double* __restrict__ red) {
+  const int64_t n = G.nc;
+  Pc p{G.nx, G.ny, G.nz};
+  double m = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x; c < n; c += (int64_t)gridDim.x * kBlk) {
+    int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+    T q[9];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = U.v[v][c];
+    q[5] = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
+    q[6] = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
+    q[7] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
+    q[8] = U.v[8][c];
+    U.v[5][c] = q[5];
+    U.v[6][c] = q[6];
+    U.v[7][c] = q[7];
+    m = nanmax(m, (double)cell_wavespeed(q, P));
+  }
+  KSum none[1];
+  block_fold<0, 1>(none, &m);
+  store_partial<0, 1>(red, none, &m);
+}
+
+template <class T>
+void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st,
+                      const StepParams<T>* P, double* red, double* dsmax) {
   dim3 blk(kBlk), grd((unsigned)((G.nc + kBlk - 1) / kBlk));
   hipLaunchKernelGGL((k_ct_emf<T>), grd, blk, 0, st, ctp, G);
   if (compensate) hipLaunchKernelGGL((k_ct_faces<T, true>), grd, blk, 0, st, ctp, G, dtdx);
   else hipLaunchKernelGGL((k_ct_faces<T, false>), grd, blk, 0, st, ctp, G, dtdx);
-  hipLaunchKernelGGL((k_ct_bcc<T>), grd, blk, 0, st, U, ctp, G);
+  if (P && red && dsmax) {
+    const unsigned nb = red_grid(G.nc);
+    hipLaunchKernelGGL((k_ct_bcc_cfl<T>), dim3(nb), blk, 0, st, U, ctp, G, *P, red);
+    hipLaunchKernelGGL((k_fold_final<0, 1>), dim3(1), blk, 0, st, red, (int)nb, dsmax);
+  } else {
+    hipLaunchKernelGGL((k_ct_bcc<T>), grd, blk, 0, st, U, ctp, G);
+  }
 }
 
 // ct.jl:176-185: exact face divergence, maximum.
@@ -368,7 +403,8 @@ void launch_ct_curl(const T* Ax, const T* Ay, const T* Az, T* bx, T* by, T* bz, 
                                     hipStream_t);                                                               \
   template void launch_divb<T>(Fields<const T>, const Grid&, T, double*, double*, hipStream_t);                  \
   template void launch_totals<T>(Fields<const T>, const Grid&, double*, double*, hipStream_t);                  \
-  template void launch_ct_update<T>(Fields<T>, T*, const Grid&, T, bool, hipStream_t);                           \
+  template void launch_ct_update<T>(Fields<T>, T*, const Grid&, T, bool, hipStream_t, const StepParams<T>*, double*, \
+                                     double*);                                                                     \
   template void launch_divb_face<T>(const T*, const Grid&, T, double*, double*, hipStream_t);                    \
   template void launch_ct_install<T>(Fields<T>, T*, const Grid&, hipStream_t);                                   \
   template void launch_ct_curl<T>(const T*, const T*, const T*, T*, T*, T*, const Grid&, T, hipStream_t);

@@ -213,6 +213,40 @@ def test_ct_float64_matches_oracle_and_fork_port():
         assert np.max(np.abs(g.ct_slot(d) - o.ct[d])) <= 1e-12 * np.abs(o.ct[d]).max()
     assert M.max_divb_face(g) / N < 1e-12
     assert np.all(Ug[8] == 0)
+    # the cell-field pass folded the finished state's CFL speed (timestep.jl:8-12)
+    _, sm = M.compute_dt(g, 0.3)
+    assert abs(sm - o.max_wavespeed()) <= 1e-14 * o.max_wavespeed()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ct_step_cfl_fold_equals_a_fresh_reduction(dtype):
+    """After a CT step the library keeps the CFL speed its cell-field pass reduced; it is
+    the speed a separate pass over the new state gives (to the last bit but for FMA
+    contraction, which may differ between the two kernels), and a speed of other arrays
+    does not leave a stale value."""
+    tol = 4e-7 if dtype == np.float32 else 1e-15
+    N = 24
+    s = M.allocate_state((N, N, N), 1.0 / N, dtype=dtype, riemann="hlld", recon="ppm", mhd_solver="ct")
+    M.init_turb_field(s)
+    M.ct_faces_from_cells(s)
+    for _ in range(3):
+        dt, _ = M.compute_dt(s, 0.3)
+        M.step(s, dt)
+        _, folded = M.compute_dt(s, 0.3)
+        s.device_ptr(0)  # invalidates the kept speed: the next call reduces the fields again
+        _, fresh = M.compute_dt(s, 0.3)
+        assert abs(folded - fresh) <= tol * fresh
+    # a field-less workspace's speed of other arrays must not stand in for the state's
+    g = M.allocate_state((16, 16, 16), 1 / 16, dtype=dtype)
+    g.set_conserved(smooth_state((16, 16, 16), seed=3))
+    dt, sm = M.compute_dt(g, 0.3)
+    M.step(g, dt, ch=sm, integrator="cube")
+    _, kept = M.compute_dt(g, 0.3)
+    other = M.allocate_state((16, 16, 16), 1 / 16, dtype=dtype)
+    other.set_conserved(2.0 * smooth_state((16, 16, 16), seed=4))
+    M.max_wavespeed_of(g, [other.device_ptr(v) for v in range(9)])
+    _, again = M.compute_dt(g, 0.3)
+    assert abs(again - kept) <= tol * kept
 
 
 def test_ct_float32_divergence_preserved_and_faces_from_cells():
@@ -265,6 +299,40 @@ def test_ct_fused_kernel_matches_oracle_and_per_cell_kernel_float64(recon, riema
         assert np.max(np.abs(a - b)) <= 1e-12 * max(np.abs(b).max(), 1e-300), q
     assert M.max_divb_face(g) / N < 1e-12
     assert np.all(Ug[8] == 0)
+    # the cell-field pass folded the finished state's CFL speed (timestep.jl:8-12)
+    _, sm = M.compute_dt(g, 0.3)
+    assert abs(sm - o.max_wavespeed()) <= 1e-14 * o.max_wavespeed()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ct_step_cfl_fold_equals_a_fresh_reduction(dtype):
+    """After a CT step the library keeps the CFL speed its cell-field pass reduced; it is
+    the speed a separate pass over the new state gives (to the last bit but for FMA
+    contraction, which may differ between the two kernels), and a speed of other arrays
+    does not leave a stale value."""
+    tol = 4e-7 if dtype == np.float32 else 1e-15
+    N = 24
+    s = M.allocate_state((N, N, N), 1.0 / N, dtype=dtype, riemann="hlld", recon="ppm", mhd_solver="ct")
+    M.init_turb_field(s)
+    M.ct_faces_from_cells(s)
+    for _ in range(3):
+        dt, _ = M.compute_dt(s, 0.3)
+        M.step(s, dt)
+        _, folded = M.compute_dt(s, 0.3)
+        s.device_ptr(0)  # invalidates the kept speed: the next call reduces the fields again
+        _, fresh = M.compute_dt(s, 0.3)
+        assert abs(folded - fresh) <= tol * fresh
+    # a field-less workspace's speed of other arrays must not stand in for the state's
+    g = M.allocate_state((16, 16, 16), 1 / 16, dtype=dtype)
+    g.set_conserved(smooth_state((16, 16, 16), seed=3))
+    dt, sm = M.compute_dt(g, 0.3)
+    M.step(g, dt, ch=sm, integrator="cube")
+    _, kept = M.compute_dt(g, 0.3)
+    other = M.allocate_state((16, 16, 16), 1 / 16, dtype=dtype)
+    other.set_conserved(2.0 * smooth_state((16, 16, 16), seed=4))
+    M.max_wavespeed_of(g, [other.device_ptr(v) for v in range(9)])
+    _, again = M.compute_dt(g, 0.3)
+    assert abs(again - kept) <= tol * kept
 
 
 def test_ct_fused_kernel_float32_matches_per_cell_kernel():
