@@ -21,6 +21,7 @@
 // once, and the global state is read once and written once per step. Workgroups are
 // remapped so that consecutive tiles land on the same XCD (and its L2), which then
 // serves the halo re-reads.
+#include <cstdlib>
 #include <type_traits>
 
 #include "launch.hpp"
@@ -708,8 +709,14 @@ void launch_plain_narrow(dim3 grd, hipStream_t st, const Fields<const T>& U, con
     launch_fused_variant<T, RECON, RS, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr);
 }
 
-// Fields of nc cells addressable with 32-bit byte offsets.
-template <class T> inline bool narrow_fields(int64_t nc) { return (uint64_t)nc * sizeof(T) < (1ull << 32); }
+// Fields of nc cells addressable with 32-bit byte offsets. PMFMHD_WIDE_OFFSETS=1 in the
+// environment selects the 64-bit-offset kernels for every grid (they otherwise run only
+// for float fields of 4 GiB and more, 1024^3 and up), so that tests can compare the two.
+template <class T> inline bool narrow_fields(int64_t nc) {
+  const char* wide = std::getenv("PMFMHD_WIDE_OFFSETS");
+  if (wide && wide[0] == '1') return false;
+  return (uint64_t)nc * sizeof(T) < (1ull << 32);
+}
 
 __global__ __launch_bounds__(256) void k_fold_max_flat(const double* __restrict__ red, int nb, double* __restrict__ out) {
   __shared__ double sh[4];

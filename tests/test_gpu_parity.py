@@ -427,6 +427,26 @@ def test_fused_float32_config1_gates():
     assert max(errs) <= 1e-5, errs
 
 
+@pytest.mark.parametrize("recon", ["plm_minmod", "ppm"])
+def test_fused_float32_wide_offsets_equal_narrow_offsets(recon, monkeypatch):
+    """Float fields below 4 GiB run the kernels with 32-bit byte offsets; 1024^3 and larger
+    grids run the 64-bit-offset instances of the same code. Forced onto a small grid, the
+    wide instances give the narrow ones' results bit for bit, the CFL fold included."""
+    dims = (48, 26, 20)
+    U = smooth_state(dims, seed=21)
+    out = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("PMFMHD_WIDE_OFFSETS", wide)
+        s = M.allocate_state(dims, 1.0 / 48, dtype=np.float32, recon=recon)
+        s.set_conserved(U)
+        dt, sm = M.compute_dt(s, 0.3)
+        for _ in range(3):
+            M.step(s, float(dt), ch=float(sm), integrator="cube")
+        out.append((s.conserved(), M.compute_dt(s, 0.3)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 def test_fused_rejects_a_one_cell_reflecting_axis():
     b = M.allocate_state((16, 16, 1), 1 / 16, bcs=("periodic", "periodic", "reflecting"))
     with pytest.raises(M.MHDError, match="reflecting axes two"):
