@@ -179,6 +179,12 @@ void step_fields_src(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, do
     bool done = launch_step_fused<T>(in, out, s->G, P, s->red, s->dval, s->st, s->zc, &S);
     mark(s);
     if (!done) throw Error(std::string(what) + ": fused kernel unavailable for this configuration");
+  } else if (ct && !s->ct_percell && S.check != CHECK_MASKED && S.check != CHECK_GLOBAL &&
+             fused_supported(s->G, s->p.precision, P.recon, P.rsolve) && all_periodic(s->G)) {
+    // the CT :ref step with sources, each face solved once
+    mark(s);
+    launch_step_fused_ct<T>(in, out, ctp, s->G, P, s->st, s->zc, &S);
+    mark(s);
   } else {
     mark(s);
     launch_step_ref<T>(in, out, ctp, s->G, P, ct, s->st, &S);

@@ -30,10 +30,11 @@ bool all_periodic(const Grid& G);
 bool surrogate_grid(const Grid& G);
 // The fused CT step (constrained-transport states, the :ref algorithm with each face
 // solved once): the cell update into O and workspace slots 3-14 of ctp; ct_update! then
-// finishes the step. Returns false when the grid shape is not supported.
+// finishes the step. src: the drag / radiation sources (no fallback masks: those belong
+// to the :cube integrator). Returns false when the grid shape is not supported.
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
-                          hipStream_t st, int zc = 0);
+                          hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
 // The surrogate-assisted fused step (float32, HLLD): simg is the network's LDS image,
 // nsmooth accumulates the number of faces solved by the network.
 bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,

@@ -156,7 +156,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
                                         SurrQueue<FQCAP>* Qp = nullptr, T* __restrict__ ctp = nullptr) {
-  static_assert(!(CT && (SURR || SRC)), "the fused CT step has no surrogate or source terms");
+  static_assert(!(CT && SURR), "the fused CT step has no surrogate");
   static_assert(!(BCG && (CT || SURR)), "the CT and surrogate steps run on periodic grids");
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
@@ -257,7 +257,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           U0[v] = o[v];
           du[v] = P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
         }
-        src_update(U0, du, o, src, idx, P, false);
+        src_update(U0, du, o, src, idx, P, CT);
       } else {
 #pragma unroll
         for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
@@ -413,13 +413,21 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         for (int v = 0; v < 9; ++v) Lzn[v] = mh[v] + dRz[v];
       }
       if constexpr (CT) {
-        // cell EMF -v x B from the predicted velocity and the start-of-step field
+        // cell EMF -v x B from the predicted velocity (SRC: with the drag propagation and
+        // radiation correction of a zero-offset face state) and the start-of-step field
         // (integrator_ref.jl:133-138), for the chunk's own cells
         if (stores && !first && !last) {
           const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
-          ctp[12 * n + idx] = mh[3] * m0[6] - mh[2] * m0[7];
-          ctp[13 * n + idx] = mh[1] * m0[7] - mh[3] * m0[5];
-          ctp[14 * n + idx] = mh[2] * m0[5] - mh[1] * m0[6];
+          T qc[9];
+#pragma unroll
+          for (int v = 0; v < 9; ++v) qc[v] = mh[v];
+          if constexpr (SRC) {
+            const T zero[9] = {};
+            src_face(qc, m0, zero, src, pcell);
+          }
+          ctp[12 * n + idx] = qc[3] * m0[6] - qc[2] * m0[7];
+          ctp[13 * n + idx] = qc[1] * m0[7] - qc[3] * m0[5];
+          ctp[14 * n + idx] = qc[2] * m0[5] - qc[1] * m0[6];
         }
       }
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
@@ -1034,17 +1042,17 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 // the face fields, integrator_ref.jl:83-139) with each face solved once; writes the cell
 // update and workspace slots 3-14 (face mass and transverse-B fluxes of every cell's
 // lower faces, cell EMFs) that ct_update! reads.
-template <class T, int RECON, int RS>
+template <class T, int RECON, int RS, bool SRC = false>
 __device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, const Fields<T>& O, T* __restrict__ ctp,
-                                                   const FusedGeom& g, const StepParams<T>& P) {
-  __shared__ FusedLds<T> lds;
+                                                   const FusedGeom& g, const StepParams<T>& P,
+                                                   const SrcParams<T>& src = SrcParams<T>{}) {
+  __shared__ FusedLds<T, SRC> lds;
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
   unsigned none = 0;
-  const SrcParams<T> nosrc{};
-  (void)fused_item<T, RECON, RS, false, false, false, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, nosrc,
-                                                            nullptr, ctp);
+  (void)fused_item<T, RECON, RS, false, SRC, false, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, src,
+                                                          nullptr, ctp);
 }
 
 template <class T, int RECON, int RS>
@@ -1067,15 +1075,26 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) v
   step_fused_ct_body<float, RECON, RS>(U, O, ctp, g, P);
 }
 
+// The CT step with the drag / radiation sources (step_ref_drag! / step_ref_radiation! on
+// CT states, integrator_ref.jl:222-258): source-modified face states and cell EMFs,
+// tracked updates (no cell-B tracking under CT), the admissibility mark.
+template <class T, int RECON, int RS>
+__global__ __launch_bounds__(FTHREADS) void k_step_fused_ct_src(Fields<const T> U, Fields<T> O, T* __restrict__ ctp,
+                                                                FusedGeom g, StepParams<T> P, SrcParams<T> src) {
+  step_fused_ct_body<T, RECON, RS, true>(U, O, ctp, g, P, src);
+}
+
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
-                          hipStream_t st, int zc) {
+                          hipStream_t st, int zc, const SrcParams<T>* src) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve) || !all_periodic(G)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
 #define PMF_CT(R, S)                                                                   \
   do {                                                                                 \
-    if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>())                           \
+    if (src)                                                                           \
+      hipLaunchKernelGGL((k_step_fused_ct_src<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P, *src); \
+    else if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>())                      \
       hipLaunchKernelGGL((k_step_fused_ct3<R, S>), grd, blk, 0, st, U, O, ctp, g, P);  \
     else                                                                               \
       hipLaunchKernelGGL((k_step_fused_ct<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P); \
@@ -1097,9 +1116,9 @@ bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G,
 }
 
 template bool launch_step_fused_ct<float>(Fields<const float>, Fields<float>, float*, const Grid&,
-                                          const StepParams<float>&, hipStream_t, int);
+                                          const StepParams<float>&, hipStream_t, int, const SrcParams<float>*);
 template bool launch_step_fused_ct<double>(Fields<const double>, Fields<double>, double*, const Grid&,
-                                           const StepParams<double>&, hipStream_t, int);
+                                           const StepParams<double>&, hipStream_t, int, const SrcParams<double>*);
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
                                        double*, double*, hipStream_t, int, const SrcParams<float>*);

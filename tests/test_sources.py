@@ -340,6 +340,57 @@ def test_ct_radiation_step_matches_oracle_float64():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("recon,riemann", [("plm", "hlld"), ("ppm", "hll"), ("plm_minmod", "llf")])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ct_coupled_steps_fused_equal_per_cell_kernel(recon, riemann, dtype):
+    """step_ref_drag! / step_ref_radiation! on CT states run on the fused CT kernel (every
+    face once) and on the per-cell kernel (pmf_set_ct_kernel): the same cells, faces,
+    workspace slots (face fluxes, cell EMFs from the source-modified velocity) and
+    correction-buffer slots, to round-off; float64 also against the oracle."""
+    import pmfmhd as M
+    from test_oracle import ct_test_hydro, ct_test_potential
+    n = 16
+    U = ct_test_hydro(n, G)
+    A = ct_test_potential(n)
+    runs = []
+    for per_cell in (False, True):
+        s = M.allocate_state((n,) * 3, 1.0 / n, dtype=dtype, recon=recon, riemann=riemann, mhd_solver="ct")
+        s.set_conserved(U)
+        M.ct_faces_from_potential(s, *A)
+        M.set_ct_kernel(s, per_cell)
+        c = _corr(n, U, seed=7, dtype=dtype)
+        c[0] = 0.0
+        buf = M.correction_buffer(s, c)
+        dt, _ = M.compute_dt(s, 0.3)
+        M.step_drag(s, float(dt), 0.0, 1.0, 0.3, integrator="ref")
+        for _ in range(2):
+            M.step_radiation(s, float(dt), 0.0, 1.0, 0.2, buf, integrator="ref")
+        runs.append((s.conserved(), [s.ct_slot(k) for k in range(15)],
+                     buf.download(np.float64 if dtype == np.float64 else np.float32, (9, n ** 3)), float(dt)))
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    (Uf, Sf, Bf, dtf), (Ur, Sr, Br, _) = runs
+    assert _rel(Uf, Ur) < tol
+    for k in range(15):
+        assert _rel(Sf[k][None], Sr[k][None]) < tol, k
+    # slot 4 accumulates density increments, differences of face fluxes: in float32 the
+    # two kernels' roundings of those fluxes show there at a few 1e-6 of its largest value
+    assert _rel(Bf[1:5], Br[1:5]) < (tol if dtype == np.float64 else 2e-5)
+    if dtype == np.float64:
+        o = O.OracleState((n,) * 3, 1.0 / n, recon=recon, riemann=riemann, mhd_solver="ct")
+        o.set_conserved(U)
+        o.ct_faces_from_potential(*A)
+        c = _corr(n, U, seed=7)
+        c[0] = 0.0
+        o.step_sources(dtf, 0.0, 1.0, drag_impulse=0.3)
+        for _ in range(2):
+            o.step_sources(dtf, 0.0, 1.0, drag_impulse=0.2, correction=c, track_density=True, track_magnetic=True,
+                           track_energy=True)
+        assert _rel(Uf, o.U) < 1e-12
+        for d in range(3):
+            assert _rel(Sf[d][None], o.ct[d][None]) < 1e-12
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["check", "fallback", "global", "robust"])
 def test_cube_radiation_checks_and_fallbacks_match_oracle(mode):
     import pmfmhd as M
