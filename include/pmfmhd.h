@@ -133,9 +133,9 @@ int pmf_set_chunk(pmf_state* s, int32_t zc);
  * CT, the per-step workspace slots) and keeps the fastest; returns it and its time per
  * step. */
 int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
-/* Kernel of the CT (:ref) step. By default a constrained-transport state steps on the
- * fused z-marching kernel, which solves every face once with the :ref algorithm
- * (integrator_ref.jl:83-139); per_cell != 0
+/* Kernel of the CT (:ref) step. By default a constrained-transport state steps (plain,
+ * drag and radiation steps alike) on the fused z-marching kernel, which solves every face
+ * once with the :ref algorithm (integrator_ref.jl:83-139, 222-258); per_cell != 0
  * selects the per-cell kernel of integrator_ref.jl:189-220 instead. No counterpart in the
  * reference (one kernel there); results agree to rounding. */
 int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell);
