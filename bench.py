@@ -23,6 +23,9 @@ for p in (os.path.join(ROOT, "pmf-glm_amd"), os.path.join(ROOT, "oracle")):
 METRIC = "cell-updates/s at 256³ FP32: surrogate vs plain HLLD at rel-L2≤1e-4 vs FP64"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 72  # 9 float32 read + 9 float32 written per cell-update
+# VALU issue capacity without packed FP32: 256 CUs x 64 lanes (four 16-lane SIMDs) per
+# cycle at the 2.4 GHz peak engine clock
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
 def log(msg):
@@ -295,6 +298,26 @@ def refbench_leg(M, P, n, iters, solver):
             "finite": bool(all(np.isfinite(a).all() for a in h)), "positive_density": bool(h[0].min() > 0)}
 
 
+def valu_from_profile(n, integ, kernel_s):
+    """VALU lane operations per cell-update from the committed SQ counter summary, and the
+    issue rate they imply at the measured kernel time."""
+    path = os.path.join(ROOT, "profiles", "k_step_fused_256cubed_sq_counters.md")
+    if n != 256 or integ != "cube" or not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0] in ("lane_ops_per_cell", "issue_slots_per_cell"):
+            vals[line.split(":")[0]] = float(line.split(":")[1])
+    if len(vals) < 2:
+        return None
+    ach = vals["issue_slots_per_cell"] * n ** 3 / kernel_s
+    return {"bound": "valu-issue", "lane_ops_per_cell": vals["lane_ops_per_cell"],
+            "issue_slots_per_cell": vals["issue_slots_per_cell"], "achieved": ach, "peak": VALU_PEAK_LANE_OPS,
+            "unit": "lane-issue-slots/s", "frac": ach / VALU_PEAK_LANE_OPS,
+            "note": "counted at 256^3: SQ_INSTS_VALU x 64 / cells, transcendentals at four slots; peak at the "
+                    "2.4 GHz engine clock, which the chip does not hold under this load (2.07 GHz measured)"}
+
+
 def traffic_from_profile(n, integ):
     """HBM bytes per fused launch from the committed FETCH_SIZE / WRITE_SIZE summary."""
     path = os.path.join(ROOT, "profiles", "k_step_fused_256cubed_traffic.md")
@@ -456,6 +479,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n, integ),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_launch": BYTES_PER_CELL * cells},
+        "valu_roofline": valu_from_profile(n, integ, avg_kernel_s),
         "cpu_baseline": cpu,
         "error_report": errors,
         "surrogate": sur,
