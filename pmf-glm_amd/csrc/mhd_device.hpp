@@ -96,7 +96,7 @@ template <class T> __device__ __forceinline__ void prim2cons(const T* q, T* u, c
 template <class T> __device__ __forceinline__ T cfast(T ir, T p, T b2, T bn, T gamma) {
   T c2 = gamma * p * ir;
   T d2 = T(0.5) * (b2 * ir + c2);
-  return dsqrt(d2 + dsqrt(dmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
+  return dsqrt(d2 + dsqrt(dmax(d2 * d2 - c2 * (bn * bn) * ir, T(0))));  // bn*bn: shared with the caller's
 }
 
 // Physical flux along axis D (glm_physics.jl:28-34 composed with the cyclic rotations
@@ -247,27 +247,29 @@ __device__ __forceinline__ void hlld_n(const T* L, const T* R, T bn, T fbn, T fp
   T SLs = SM - abn * rqL, SRs = SM + abn * rqR;
   // single-star transverse states (glm_physics.jl:255-263)
   T vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
+  // a degenerate star state (|denK| < 1e-12) keeps the side's transverse velocity and
+  // field: fac = 0 and gg = 1 give exactly that, with two selects per side instead of four
   {
     T denK = aL * (SL - SM) - bn2;
     bool deg = dabs(denK) < T(1e-12);
     T ik = rcp(denK);
-    T fac = bn * (SM - uL) * ik;
-    T gg = (aL * (SL - uL) - bn2) * ik;
-    vyLs = deg ? vL : vL - byL * fac;
-    vzLs = deg ? wL : wL - bzL * fac;
-    byLs = deg ? byL : byL * gg;
-    bzLs = deg ? bzL : bzL * gg;
+    T fac = deg ? T(0) : bn * (SM - uL) * ik;
+    T gg = deg ? T(1) : (aL * (SL - uL) - bn2) * ik;
+    vyLs = vL - byL * fac;
+    vzLs = wL - bzL * fac;
+    byLs = byL * gg;
+    bzLs = bzL * gg;
   }
   {
     T denK = aR * (SR - SM) - bn2;
     bool deg = dabs(denK) < T(1e-12);
     T ik = rcp(denK);
-    T fac = bn * (SM - uR) * ik;
-    T gg = (aR * (SR - uR) - bn2) * ik;
-    vyRs = deg ? vR : vR - byR * fac;
-    vzRs = deg ? wR : wR - bzR * fac;
-    byRs = deg ? byR : byR * gg;
-    bzRs = deg ? bzR : bzR * gg;
+    T fac = deg ? T(0) : bn * (SM - uR) * ik;
+    T gg = deg ? T(1) : (aR * (SR - uR) - bn2) * ik;
+    vyRs = vR - byR * fac;
+    vzRs = wR - bzR * fac;
+    byRs = byR * gg;
+    bzRs = bzR * gg;
   }
   // double-star state (glm_physics.jl:288-296): needs both single-star fans
   T sg = bn >= T(0) ? T(1) : T(-1);
