@@ -164,7 +164,20 @@ def surrogate_leg(M, P, n, steps, weights, thr):
             "network_face_fraction_first_last": [c / (3 * cells) for c in counts],
             "rel_l2_vs_fp64": {"plain": e_plain, "surrogate": e_sur},
             "meets_rel_l2_1e-4": bool(e_sur["all_but_psi"] <= 1e-4),
-            "gate_sweep": sweep}
+            "gate_sweep": sweep,
+            "fastest_meeting_rel_l2_1e-4": fastest_meeting(cells * steps / t_sur, e_sur["all_but_psi"], thr, sweep,
+                                                           cells * steps / t_plain)}
+
+
+def fastest_meeting(value, err, thr, sweep, plain_value):
+    """The fastest gate whose whole-run error meets the 1e-4 target, against plain."""
+    rows = [{"thr": thr, "value": value, "rel_l2_all_but_psi": err}] + \
+        [{"thr": r["thr"], "value": r["value"], "rel_l2_all_but_psi": r["rel_l2_all_but_psi"]} for r in sweep]
+    ok = [r for r in rows if r["rel_l2_all_but_psi"] <= 1e-4]
+    if not ok:
+        return None
+    best = max(ok, key=lambda r: r["value"])
+    return dict(best, plain_value=plain_value, speedup_vs_plain=best["value"] / plain_value)
 
 
 def ensemble_leg(P):
