@@ -18,9 +18,9 @@ def _members(edges):
     return out
 
 
-def _single_run(U, n, steps, cfl=0.2, recon="plm_minmod"):
+def _single_run(U, n, steps, cfl=0.2, recon="plm_minmod", riemann="hlld"):
     import pmfmhd as M
-    s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon=recon)
+    s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon=recon, riemann=riemann)
     s.set_conserved(U)
     t = 0.0
     for _ in range(steps):
@@ -78,6 +78,26 @@ def test_ensemble_matches_single_cube_runs_graph_and_eager():
         print(f"cube {c} (n={n}): max |ensemble - single| / max = {d:.3e}, t {outs[True][1][0][c]!r} vs {t!r}")
         assert d == 0.0, f"cube {c}: {d}"
         assert outs[True][1][0][c] == t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("recon,riemann", [("ppm", "hlld"), ("plm", "hll"), ("plm_minmod", "llf")])
+def test_ensemble_other_schemes_match_single_cube_runs(recon, riemann):
+    """Every reconstruction and Riemann solver of the ensemble (its own kernel instances)
+    against separate single-state runs, bit for bit."""
+    from pmfmhd.ensemble import Ensemble
+    edges = [32, 16, 48]
+    mem = _members(edges)
+    Us = [P.ensemble_state(m) for m in mem]
+    e = Ensemble(edges, recon=recon, riemann=riemann)
+    for c, U in enumerate(Us):
+        e.upload(c, U)
+    e.run(5)
+    t_ens = e.clock()[0]
+    for c, n in enumerate(edges):
+        ref, t = _single_run(Us[c], n, 5, recon=recon, riemann=riemann)
+        assert np.array_equal(e.download(c), ref), (c, recon, riemann)
+        assert t_ens[c] == t
 
 
 @pytest.mark.gpu
