@@ -269,7 +269,8 @@ def ct_leg(M, P, n, steps):
 def refbench_leg(M, P, n, iters, solver):
     """The reference's own benchmark (scripts/benchmark.jl:41-72): float32, PPM, HLLD,
     the init_turb_field! state (problems.jl:9-29), one dt and smax from compute_dt at CFL
-    0.2, then `iters` blocking steps with that dt (ch = smax; CT ignores it) on the GPU
+    0.2, then `iters` blocking steps with that dt (ch = smax; CT ignores it; each step waits
+    for the GPU, as the reference's does) on the GPU
     default integrator (GLM: cube; CT: ref), N^3 iters / wall time. The recorded figures
     are the reference's only GPU throughput numbers (Apple M5 Max, Metal)."""
     import numpy as np
@@ -286,10 +287,12 @@ def refbench_leg(M, P, n, iters, solver):
     dt, sm = M.compute_dt(s, 0.2)
 
     def advance():
+        # blocking like the reference's step! (its launchers end with KA.synchronize)
         if solver == "ct":
             M.step(s, float(dt), integrator=integ)
         else:
             M.step(s, float(dt), ch=float(sm), integrator=integ)
+        s.synchronize()
 
     advance()
     init()  # the warm-up must not advance the measured state (benchmark.jl:50)
