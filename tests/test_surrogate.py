@@ -208,6 +208,30 @@ def test_dump_inputs_and_targets_match_oracle_encoding(gpu_state):
 
 
 @pytest.mark.gpu
+def test_surrogate_step_rejects_what_it_does_not_cover():
+    """The learned flux stands in for HLLD on float32 GLM states of the fused kernel's whole
+    tiles on periodic grids; everything else is an error, never a silent plain step."""
+    import pmfmhd as M
+    from pmfmhd.surrogate import Surrogate, step_surrogate
+    cases = [
+        (dict(dims=(24, 16, 8)), "all-periodic grid whose nx is a multiple of 16"),
+        (dict(dims=(16, 16, 8), bcs=("outflow", "periodic", "periodic")), "all-periodic grid"),
+        (dict(dims=(16, 16, 8), riemann="hll"), "stands in for HLLD"),
+    ]
+    for kw, msg in cases:
+        dims = kw.pop("dims")
+        s = M.allocate_state(dims, 1.0 / dims[0], dtype=np.float32, **kw)
+        s.set_conserved(np.tile(np.array([1.0, 0, 0, 0, 1.5, 0.1, 0, 0, 0])[:, None], (1, int(np.prod(dims)))))
+        net = Surrogate(s, seed=1)
+        with pytest.raises(M.MHDError, match=msg):
+            step_surrogate(s, net, 1e-3, 1.0)
+    s = M.allocate_state((16, 16, 8), 1 / 16, dtype=np.float32)
+    net = Surrogate(s, seed=1)
+    with pytest.raises(M.MHDError, match="GLM cleaning speed"):
+        step_surrogate(s, net, 1e-3, 0.0)
+
+
+@pytest.mark.gpu
 def test_surrogate_step_without_gated_faces_equals_plain_fused_step():
     import pmfmhd as M
     from pmfmhd import problems as P
