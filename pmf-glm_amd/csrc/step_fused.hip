@@ -134,11 +134,22 @@ __device__ __forceinline__ int ylidx(int ix, int iy) { return (iy + 1) * TX + ix
 
 // LDS of one workgroup of the fused step (37.9 KB without sources: four workgroups fit
 // a CU's 160 KB).
-template <class T, bool SRC = false>
+// Where the x/y divergence of a column is carried to the next plane: in LDS for PPM,
+// whose edge offsets fill the three-wave register budget (it freed 8 VGPRs and the 16-20
+// spilled dwords: 256^3 PPM-HLLD 0.99 -> 0.95 ms, PPM-HLL 0.88 -> 0.82 ms), in registers
+// for PLM, where the extra LDS traffic cost 1-2%, and for the surrogate step, whose LDS
+// is full.
+template <int RECON, bool SURR>
+constexpr bool dxy_in_lds() {
+  return RECON == RECON_PPM && !SURR;
+}
+
+template <class T, bool SRC = false, bool DXL = true>
 struct FusedLds {
   T sP[2][9 * FNP];  // primitive planes m (current) and m-1, alternating
   T sLX[9 * FNFX];   // left states of the x faces, then the x-face fluxes
   T sLY[9 * FNFY];   // left states of the y faces, then the y-face fluxes
+  T sD[DXL ? 9 * NOWN : 1];  // DXL: each owned column's x/y divergence, carried a plane
   unsigned char sMk[SRC ? FNQ : 1];  // masked-fallback flags of the predicted cells
 };
 
@@ -152,7 +163,7 @@ struct FusedLds {
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false,
           bool BCG = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
-                                        const StepParams<T>& P, int w, FusedLds<T, SRC>& S,
+                                        const StepParams<T>& P, int w, FusedLds<T, SRC, dxy_in_lds<RECON, SURR>()>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
                                         SurrQueue<FQCAP>* Qp = nullptr, T* __restrict__ ctp = nullptr) {
@@ -216,11 +227,18 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   const int me = pidx(ix, iy);
 
   // Column registers: Lzp = left state of the z face below the current plane, Fzp = flux
-  // of the z face below the plane being updated, Dxy = that plane's x/y divergence.
+  // of the z face below the plane being updated, Dxy = that plane's x/y divergence (PPM:
+  // in the thread's own LDS slots, nine registers fewer across the prediction and the z
+  // face).
   // The LDS buffer of plane m-1 is refilled with plane m+1 while plane m is processed:
   // each thread overwrites its own column slot (read only by itself, for the z slope)
   // after its prediction, and the extra cells' slots are never read during plane m.
-  T Lzp[9], Fzp[9], Dxy[9];
+  constexpr bool DXL = dxy_in_lds<RECON, SURR>();
+  T Lzp[9], Fzp[9], Dxy[DXL ? 1 : 9];
+  auto dxy = [&](int v) -> T& {
+    if constexpr (DXL) return S.sD[v * NOWN + t];
+    else return Dxy[v];
+  };
   {
     T c[9], q[9];
     if (has_col) {
@@ -238,7 +256,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   for (int v = 0; v < 9; ++v) {
     Lzp[v] = T(0);
     Fzp[v] = T(0);
-    Dxy[v] = T(0);
+    if (owned) dxy(v) = T(0);
   }
   T smax = T(0);
 
@@ -255,12 +273,12 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
           U0[v] = o[v];
-          du[v] = P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+          du[v] = P.dtdx * (dxy(v) + (Fzp[v] - Fz[v]));
         }
         src_update(U0, du, o, src, idx, P, CT);
       } else {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (Dxy[v] + (Fzp[v] - Fz[v]));
+        for (int v = 0; v < 9; ++v) o[v] += P.dtdx * (dxy(v) + (Fzp[v] - Fz[v]));
       }
       if constexpr (CT) {
         o[8] = T(0);  // no cleaning field under CT (integrator_ref.jl:150-158)
@@ -534,7 +552,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         if (owned) {
 #pragma unroll
           for (int v = 0; v < 9; ++v)
-            Dxy[v] = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
+            dxy(v) = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
                      (S.sLY[v * FNFY + ylidx(ix, iy - 1)] - S.sLY[v * FNFY + ylidx(ix, iy)]);
         }
       }
@@ -598,7 +616,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       if (owned) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
-          Dxy[v] = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
+          dxy(v) = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
                    (S.sLY[v * FNFY + ylidx(ix, iy - 1)] - S.sLY[v * FNFY + ylidx(ix, iy)]);
       }
     }
@@ -638,7 +656,7 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
                                                 const StepParams<T>& P, double* __restrict__ red,
                                                 const unsigned char* __restrict__ simg, float thr,
                                                 unsigned long long* __restrict__ nsmooth, const SrcParams<T>& src) {
-  __shared__ FusedLds<T, SRC> lds;
+  __shared__ FusedLds<T, SRC, dxy_in_lds<RECON, SURR>()> lds;
   __shared__ T sMax[FTHREADS / 64];
   __shared__ __attribute__((aligned(16))) unsigned short sW[SURR ? SIMG_NW : 2];
   __shared__ __attribute__((aligned(16))) SurrQueue<SURR ? FQCAP : 1> sQ;
@@ -939,7 +957,7 @@ template <int RECON, int RS>
 __device__ __forceinline__ void ens_step_body(const EnsCube* __restrict__ cubes, const int2* __restrict__ items,
                                               const EnsQueues& q, const StepParams<float>* __restrict__ params,
                                               double* __restrict__ red, unsigned* __restrict__ ctr, int parity) {
-  __shared__ FusedLds<float> lds;
+  __shared__ FusedLds<float, false, dxy_in_lds<RECON, false>()> lds;
   __shared__ float sMax[FTHREADS / 64];
   __shared__ int sItem;
   unsigned none = 0;
@@ -1046,7 +1064,7 @@ template <class T, int RECON, int RS, bool SRC = false>
 __device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, const Fields<T>& O, T* __restrict__ ctp,
                                                    const FusedGeom& g, const StepParams<T>& P,
                                                    const SrcParams<T>& src = SrcParams<T>{}) {
-  __shared__ FusedLds<T, SRC> lds;
+  __shared__ FusedLds<T, SRC, dxy_in_lds<RECON, false>()> lds;
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
