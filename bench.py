@@ -352,7 +352,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--integrator", default="auto", choices=["auto", "cube", "ref"])
-    ap.add_argument("--cpu-steps", type=int, default=1, help="steps in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="steps in the CPU-baseline sample (0 = skip; 5 steps at 256^3 are about 12 s of CPU work)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="steps of the float32-vs-float64 error report (0 = skip)")
     ap.add_argument("--surrogate-steps", type=int, default=200,
