@@ -72,14 +72,24 @@ __device__ __forceinline__ void src_update(const T* U0, const T* du, T* o, const
                                            const StepParams<T>& P, bool ct) {
   T* const cr = S.corr;
   const int64_t n = S.stride;
+  const bool trho = S.flags & SRC_TRACK_RHO;
   const bool te = S.flags & SRC_TRACK_E, tb = (S.flags & SRC_TRACK_B) && !ct;
-  if (S.flags & SRC_TRACK_RHO) cr[4 * n + idx] = cr[4 * n + idx] + du[0];
+  // The tracked slots are distinct arrays: all are read before any is written, so their
+  // loads are in flight together (one memory latency, not one per read-after-write), and
+  // the residuals are carried in registers and stored once at the end.
+  T r4 = T(0), r8 = T(0), rb[3] = {T(0), T(0), T(0)};
+  if (trho) r4 = cr[4 * n + idx];
+  if (te) r8 = cr[8 * n + idx];
+  if (tb) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rb[c] = cr[(5 + c) * n + idx];
+  }
   T rh[9];
 #pragma unroll
   for (int v = 0; v < 4; ++v) rh[v] = U0[v] + du[v];
-  rh[4] = comp_update(te ? cr + 8 * n + idx : nullptr, U0[4], du[4], te);
+  rh[4] = comp_update(&r8, U0[4], du[4], te);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) rh[5 + c] = comp_update(tb ? cr + (5 + c) * n + idx : nullptr, U0[5 + c], du[5 + c], tb);
+  for (int c = 0; c < 3; ++c) rh[5 + c] = comp_update(&rb[c], U0[5 + c], du[5 + c], tb);
   rh[8] = U0[8] + du[8];
   const bool bad = S.check != CHECK_NONE && src_inadmissible(rh, P);
 #pragma unroll
@@ -109,9 +119,9 @@ __device__ __forceinline__ void src_update(const T* U0, const T* du, T* o, const
     o[2] = nmy;
     o[3] = nmz;
     if (te) {
-      T e = comp_update(cr + 8 * n + idx, rh[4], -qwork, true);
+      T e = comp_update(&r8, rh[4], -qwork, true);
       if (e < emin) {
-        cr[8 * n + idx] = T(0);
+        r8 = T(0);
         e = emin;
       }
       o[4] = e;
@@ -120,6 +130,12 @@ __device__ __forceinline__ void src_update(const T* U0, const T* du, T* o, const
     }
   }
   if (bad) o[0] = -dmax(dabs(o[0]), dmax(P.smallr, P.llf_dmin));
+  if (trho) cr[4 * n + idx] = r4 + du[0];
+  if (te) cr[8 * n + idx] = r8;
+  if (tb) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cr[(5 + c) * n + idx] = rb[c];
+  }
 }
 
 }  // namespace pmf
