@@ -40,6 +40,23 @@ __device__ __forceinline__ void src_face(T* q, const T* m0, const T* d, const Sr
   }
 }
 
+// The same with the cell's radiation correction already loaded (cv = dv_x, dv_y, dv_z).
+template <class T>
+__device__ __forceinline__ void src_face_pre(T* q, const T* m0, const T* d, const SrcParams<T>& S, const T* cv) {
+  if (S.flags & SRC_DRAG) {
+    const T v0[3] = {S.vx0, S.vy0, S.vz0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T f0 = m0[1 + c] + d[1 + c];
+      q[1 + c] = v0[c] + S.face_decay * (f0 - v0[c]) + S.face_phi1 * (q[1 + c] - f0);
+    }
+  }
+  if (S.flags & SRC_RAD) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[1 + c] = q[1 + c] + cv[c];
+  }
+}
+
 // x + dx, carrying the rounding error in *res when tracked (Fast2Sum; glm_physics.jl:350-358).
 template <class T> __device__ __forceinline__ T comp_update(T* res, T x, T dx, bool tracked) {
   if (!tracked) return x + dx;

@@ -311,6 +311,20 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     if (has_col) load_plane(m + 1, col, 0, Pn);
     const bool extra_now = has_extra && m + 1 < z0 + zc;
     if (extra_now) load_plane(m + 1, ecol, 2, Xc);
+    // SRC: the radiation correction and the fallback mask of this plane's predicted cell,
+    // also in flight during the barrier (they are used after the prediction)
+    T cv[3] = {T(0), T(0), T(0)};
+    T mkv = T(0);
+    if constexpr (SRC) {
+      if (has_col) {
+        const int64_t pc = (int64_t)wrapi(m, g.nz) * g.plane + pcol;
+        if (src.flags & SRC_RAD) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) cv[c] = src.corr[(1 + c) * src.stride + pc];
+        }
+        if (src.check == CHECK_MASKED) mkv = src.corr[4 * src.stride + pc];
+      }
+    }
     if constexpr (SURR) {
       if (t == 0) Qp->n = 0u;  // the previous plane's queue was last read before its final barrier
     }
@@ -339,13 +353,10 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         offsets<RECON>(nxt[v * FNP + me], m0[v], Pn[v], dLz[v], dRz[v]);
         nxt[v * FNP + me] = Pn[v];
       }
-      int64_t pcell = 0;  // the predicted cell (plane m of this column)
       if constexpr (SRC) {
         // masked / global fallback: flagged cells are predicted with zero slopes
-        // (integrator_cube.jl:330-335)
-        pcell = (int64_t)wrapi(m, g.nz) * g.plane + pcol;
-        const bool mk = src.check == CHECK_GLOBAL ||
-                        (src.check == CHECK_MASKED && src.corr[4 * src.stride + pcell] > T(0));
+        // (integrator_cube.jl:330-335); mkv is the predicted cell's slot 4
+        const bool mk = src.check == CHECK_GLOBAL || (src.check == CHECK_MASKED && mkv > T(0));
         if (mk) {
 #pragma unroll
           for (int v = 0; v < 9; ++v) dLx[v] = dRx[v] = dLy[v] = dRy[v] = dLz[v] = dRz[v] = T(0);
@@ -398,12 +409,12 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           Rz[v] = mh[v] + dLz[v];
           Lzn[v] = mh[v] + dRz[v];
         }
-        src_face(fx, m0, dRx, src, pcell);
-        src_face(fy, m0, dRy, src, pcell);
-        src_face(Rx, m0, dLx, src, pcell);
-        src_face(Ry, m0, dLy, src, pcell);
-        src_face(Rz, m0, dLz, src, pcell);
-        src_face(Lzn, m0, dRz, src, pcell);
+        src_face_pre(fx, m0, dRx, src, cv);
+        src_face_pre(fy, m0, dRy, src, cv);
+        src_face_pre(Rx, m0, dLx, src, cv);
+        src_face_pre(Ry, m0, dLy, src, cv);
+        src_face_pre(Rz, m0, dLz, src, cv);
+        src_face_pre(Lzn, m0, dRz, src, cv);
         if (xleft) {
 #pragma unroll
           for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix, iy)] = fx[v];
@@ -441,7 +452,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           for (int v = 0; v < 9; ++v) qc[v] = mh[v];
           if constexpr (SRC) {
             const T zero[9] = {};
-            src_face(qc, m0, zero, src, pcell);
+            src_face_pre(qc, m0, zero, src, cv);
           }
           ctp[12 * n + idx] = qc[3] * m0[6] - qc[2] * m0[7];
           ctp[13 * n + idx] = qc[1] * m0[7] - qc[3] * m0[5];
