@@ -113,9 +113,12 @@ __device__ __forceinline__ void src_update(const T* U0, const T* du, T* o, const
   for (int v = 0; v < 9; ++v) o[v] = rh[v];
   if (S.flags & SRC_DRAG) {
     // glm_physics.jl:143-170,383-390
+    // one reciprocal per density, shared by its divisions (glm_physics.jl divides; the
+    // products differ from the quotients by about an ulp)
     const T rn = dmax(rh[0], P.smallr), ro = dmax(U0[0], P.smallr);
-    const T vox = U0[1] / ro, voy = U0[2] / ro, voz = U0[3] / ro;
-    const T dvx = rh[1] / rn - vox, dvy = rh[2] / rn - voy, dvz = rh[3] / rn - voz;
+    const T iro = rcp(ro), irn = rcp(rn);
+    const T vox = U0[1] * iro, voy = U0[2] * iro, voz = U0[3] * iro;
+    const T dvx = rh[1] * irn - vox, dvy = rh[2] * irn - voy, dvz = rh[3] * irn - voz;
     const T wx = vox - S.vx0, wy = voy - S.vy0, wz = voz - S.vz0;
     const T vnx = S.vx0 + S.decay * wx + S.phi1 * dvx;
     const T vny = S.vy0 + S.decay * wy + S.phi1 * dvy;
@@ -128,7 +131,7 @@ __device__ __forceinline__ void src_update(const T* U0, const T* du, T* o, const
     const T iwz = (T(1) - S.decay) * wz + (T(1) - S.phi1) * dvz;
     const T qwork = rn * (S.A * w2 + S.C * wdv + S.D * dv2 + S.vx0 * iwx + S.vy0 * iwy + S.vz0 * iwz);
     const T nmx = rn * vnx, nmy = rn * vny, nmz = rn * vnz;
-    const T ek = T(0.5) * (nmx * nmx + nmy * nmy + nmz * nmz) / rn;
+    const T ek = T(0.5) * (nmx * nmx + nmy * nmy + nmz * nmz) * irn;
     const T eb = T(0.5) * (rh[5] * rh[5] + rh[6] * rh[6] + rh[7] * rh[7]);
     const T emin = ek + eb + T(1e-30);
     o[0] = rn;
