@@ -266,6 +266,47 @@ def ct_leg(M, P, n, steps):
     return out
 
 
+def coupled_leg(M, U, n, steps):
+    """The drag and radiation-coupled fused steps (step_cube_drag!, step_cube_radiation!
+    with density, field and energy tracking; integrator_cube.jl:478-540), which the
+    production runner calls instead of step! (runner.jl:6660-6694), on the plain run's
+    inputs with a fixed dt, drag impulse 0.1 and a synthetic velocity correction; psi is
+    copied into the correction buffer before each radiation step, as the radiation driver
+    does (radiation_coupling.jl:1681). kernel_ms is the fused step kernel alone."""
+    import numpy as np
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(U)
+    c = np.zeros((9, n ** 3), np.float32)
+    c[1:4] = np.random.default_rng(1).normal(0, 0.01, (3, n ** 3)).astype(np.float32)
+    buf = M.correction_buffer(s, c)
+    dt, sm = M.compute_dt(s, 0.2)
+    out = {"workload": f"coupled cube steps FP32 {n}^3 PLM minmod HLLD on the plain run's inputs, "
+                       f"fixed CFL-0.2 dt, drag impulse 0.1, {steps} steps each"}
+
+    def drag():
+        M.step_drag(s, float(dt), float(sm), 0.99, 0.1, integrator="cube")
+
+    def radiation():
+        M.load_psi(s, buf)
+        M.step_radiation(s, float(dt), float(sm), 0.99, 0.1, buf, integrator="cube")
+
+    for name, fn in (("drag", drag), ("radiation", radiation)):
+        fn()
+        s.synchronize()
+        s.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        s.synchronize()
+        el = time.perf_counter() - t0
+        kms, nl = s.timing_read()
+        s.timing(False)
+        out[name] = {"value": n ** 3 * steps / el, "ms_per_step": el * 1e3 / steps,
+                     "kernel_ms": kms / max(nl, 1)}
+    out["finite"] = bool(all(np.isfinite(a).all() for a in s.fields_to_host()))
+    return out
+
+
 def refbench_leg(M, P, n, iters, solver):
     """The reference's own benchmark (scripts/benchmark.jl:41-72): float32, PPM, HLLD,
     the init_turb_field! state (problems.jl:9-29), one dt and smax from compute_dt at CFL
@@ -366,6 +407,8 @@ def main():
     ap.add_argument("--fp64-oracle-steps", type=int, default=10,
                     help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~4 s per step)")
     ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
+    ap.add_argument("--coupled-steps", type=int, default=20,
+                    help="steps of each coupled (drag, radiation) leg (0 = skip)")
     ap.add_argument("--refbench", action=argparse.BooleanOptionalAction, default=True,
                     help="the reference's own benchmark workload (scripts/benchmark.jl), GLM at N=128 and CT at N=96")
     ap.add_argument("--calib-copy", action="store_true",
@@ -470,6 +513,13 @@ def main():
     if args.ct_steps > 0:
         log(f"bench: CT leg ({args.ct_steps} steps)")
         ct = ct_leg(M, P, n, args.ct_steps)
+    coupled = None
+    if args.coupled_steps > 0 and integ == "cube":
+        log(f"bench: coupled-step leg ({args.coupled_steps} steps each)")
+        coupled = coupled_leg(M, U, n, args.coupled_steps)
+        log("bench: coupled steps " + ", ".join(f"{k} {coupled[k]['ms_per_step']:.3f} ms/step "
+                                                f"(kernel {coupled[k]['kernel_ms']:.3f} ms)"
+                                                for k in ("drag", "radiation")))
     refb = None
     if args.refbench:
         log("bench: the reference's benchmark workload")
@@ -504,6 +554,7 @@ def main():
         "c3_512": c3,
         "ct": ct,
         "fp64_oracle_check": orc,
+        "coupled_steps": coupled,
         "reference_benchmark": refb,
     }
     print(json.dumps(out), flush=True)
