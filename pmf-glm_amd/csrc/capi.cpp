@@ -253,9 +253,8 @@ int pmf_params_default(pmf_params* p, int32_t nx, int32_t ny, int32_t nz, double
 }
 
 namespace {
-void create_state(const pmf_params* pp, pmf_state** out, bool with_fields) {
-  if (!pp || !out) throw Error("null argument");
-  const pmf_params& p = *pp;
+// Parameter checks of allocate_state (state.jl:123-141) shared by every way of making a state.
+void validate_params(const pmf_params& p) {
   if (p.nx < 1 || p.ny < 1 || p.nz < 1) throw Error("grid dimensions must be positive");
   if (!(p.dx > 0)) throw Error("dx must be positive");
   if (p.riemann < 0 || p.riemann > 2) throw Error("unknown MHD riemann code (have :hlld, :hll, :llf)");
@@ -266,6 +265,12 @@ void create_state(const pmf_params* pp, pmf_state** out, bool with_fields) {
   if (p.precision != PMF_F32 && p.precision != PMF_F64) throw Error("precision must be PMF_F32 or PMF_F64");
   if (p.solver == PMF_SOLVER_CT && (p.bc[0] || p.bc[1] || p.bc[2]))
     throw Error("mhd_solver=:ct requires all-periodic boundaries; constrained transport is periodic-only");
+}
+
+void create_state(const pmf_params* pp, pmf_state** out, bool with_fields) {
+  if (!pp || !out) throw Error("null argument");
+  const pmf_params& p = *pp;
+  validate_params(p);
   auto* s = new pmf_state();
   try {
     s->p = p;
@@ -769,6 +774,7 @@ int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, vo
   return guard([&] {
     check_state(s);
     if (!U || !O) throw Error("null field array");
+    s->smax_valid = false;  // the fused epilogue leaves dval[0] describing the caller's output arrays
     SourceCall a;
     a.radiation = correction != nullptr;
     a.impulse = drag_impulse;

@@ -39,8 +39,10 @@ __device__ __forceinline__ int64_t passive_index(int i, int j, int k, int n) {
   return (int64_t)i + (int64_t)n * ((int64_t)j + (int64_t)n * k);
 }
 
-// passive_transport.jl:40-54: muladd(f, r - l, l) along x, then y, then z
-template <class T> __device__ __forceinline__ T lerp(T l, T r, T f) { return __builtin_fma(f, r - l, l); }
+// passive_transport.jl:40-54: muladd(f, r - l, l) along x, then y, then z, as one fused
+// multiply-add in the field's own precision
+__device__ __forceinline__ float lerp(float l, float r, float f) { return __builtin_fmaf(f, r - l, l); }
+__device__ __forceinline__ double lerp(double l, double r, double f) { return __builtin_fma(f, r - l, l); }
 template <class T> __device__ __forceinline__ T trilinear(const T* __restrict__ a, int i0, int j0, int k0, int n, T fx,
                                                           T fy, T fz) {
   const T p000 = a[passive_index(i0, j0, k0, n)], p100 = a[passive_index(i0 + 1, j0, k0, n)];

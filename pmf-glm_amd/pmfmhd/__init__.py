@@ -124,6 +124,17 @@ class MHDState:
         check(lib().pmf_device_ptr(self._h, var, ctypes.byref(p)))
         return p.value
 
+    def scratch_ptr(self, var: int) -> int:
+        p = ctypes.c_void_p()
+        check(lib().pmf_scratch_ptr(self._h, var, ctypes.byref(p)))
+        return p.value
+
+    def ct_ptr(self, slot: int = 0) -> int:
+        """Device address of CT slot q; slot 0 starts the 21-slot packed buffer."""
+        p = ctypes.c_void_p()
+        check(lib().pmf_ct_ptr(self._h, slot, ctypes.byref(p)))
+        return p.value
+
     def synchronize(self):
         check(lib().pmf_synchronize(self._h))
 
@@ -318,12 +329,32 @@ def step_radiation(s: MHDState, dt, ch, decay, drag_impulse, correction: DeviceB
                    global_fallback=False, robust_fallback=False, integrator="ref"):
     """step_ref_radiation! / step_cube_radiation! (integrator_ref.jl:238-258,
     integrator_cube.jl:505-540). psi is read from slot 0 of the correction buffer."""
-    flags = (1 if track_density else 0) | (2 if track_magnetic else 0) | (4 if track_energy else 0) | \
-        (8 if check_admissibility else 0) | (16 if fallback else 0) | (32 if global_fallback else 0) | \
-        (64 if robust_fallback else 0)
+    flags = source_flags(track_density, track_magnetic, track_energy, check_admissibility, fallback,
+                         global_fallback, robust_fallback)
     check(lib().pmf_step_radiation(s.handle, float(dt), float(ch), float(decay), float(drag_impulse),
                                    ctypes.c_void_p(correction.ptr), flags, _integ(integrator)))
     return s
+
+
+def source_flags(track_density=True, track_magnetic=True, track_energy=True, check_admissibility=False,
+                 fallback=False, global_fallback=False, robust_fallback=False) -> int:
+    """The PMF_SRC_* bit set of the radiation step's keyword options."""
+    return (1 if track_density else 0) | (2 if track_magnetic else 0) | (4 if track_energy else 0) | \
+        (8 if check_admissibility else 0) | (16 if fallback else 0) | (32 if global_fallback else 0) | \
+        (64 if robust_fallback else 0)
+
+
+def step_sources_into(ws: MHDState, U_ptrs, O_ptrs, dt, ch, decay, drag_impulse, target_velocity=(0.0, 0.0, 0.0),
+                      correction=None, flags=0, integrator="ref", ct_ptr=None):
+    """The _cube_drag_launch! / _cube_radiation_launch! seam (integrator_cube.jl:478-540):
+    one drag (correction None) or radiation-coupled step from nine caller device arrays into
+    nine others, no swap, asynchronous on the state's stream. correction is a DeviceBuffer
+    or an integer device address; flags a source_flags() value."""
+    v0 = (ctypes.c_double * 3)(*[float(v) for v in target_velocity])
+    corr = None if correction is None else ctypes.c_void_p(int(getattr(correction, "ptr", correction)))
+    ct = None if ct_ptr is None else ctypes.c_void_p(int(ct_ptr))
+    check(lib().pmf_step_sources_into(ws.handle, _ptrs(U_ptrs), _ptrs(O_ptrs), ct, corr, float(dt), float(ch),
+                                      float(decay), float(drag_impulse), v0, int(flags), _integ(integrator)))
 
 
 def set_chunk(s: MHDState, zc: int):

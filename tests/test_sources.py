@@ -475,3 +475,101 @@ def test_coupled_step_error_texts():
     c, _ = _pair(n, U, np.float32, "plm", "hll", solver="ct")
     with pytest.raises(M.MHDError, match="step_cube_drag!: CT is implemented on the :ref integrator only"):
         M.step_drag(c, 1e-3, 0.0, 1.0, 0.1, integrator="cube")
+
+
+def _device_fields(M, arrays):
+    bufs = []
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        b = M.DeviceBuffer(a.nbytes)
+        b.upload(a)
+        bufs.append(b)
+    return bufs
+
+
+_SEAM_CASES = {
+    # name: (solver, integrator, radiation, flags)
+    "drag-ref": ("glm", "ref", False, 0),
+    "drag-cube": ("glm", "cube", False, 0),
+    "radiation-ref-tracked": ("glm", "ref", True, 7),
+    "radiation-cube-check": ("glm", "cube", True, 7 | 8),
+    "radiation-cube-fallback": ("glm", "cube", True, 6 | 16),
+    "radiation-cube-robust": ("glm", "cube", True, 1 | 64),
+    "ct-drag": ("ct", "ref", False, 0),
+    "ct-radiation-tracked": ("ct", "ref", True, 7),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(_SEAM_CASES))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sources_into_caller_arrays_equal_the_state_steps(case, dtype):
+    """The _cube_drag_launch! / _cube_radiation_launch! seam (pmf_step_sources_into): nine
+    caller arrays stepped through a field-less workspace into nine others, with the caller's
+    correction buffer and, for CT, its 21-slot packed buffer, give the same bits as
+    pmf_step_drag / pmf_step_radiation on a state holding the same fields: cells, correction
+    slots and CT slots alike."""
+    import pmfmhd as M
+    from test_oracle import ct_test_hydro, ct_test_potential
+    solver, integ, radiation, flags = _SEAM_CASES[case]
+    n = 16
+    ct = solver == "ct"
+    riemann = "hll" if ct else "hlld"
+    U = ct_test_hydro(n, G) if ct else smooth_state(n, seed=13)
+    s, _ = _pair(n, U, dtype, "plm", riemann, solver=solver)
+    if ct:
+        M.ct_faces_from_potential(s, *ct_test_potential(n))
+    dt, smax = M.compute_dt(s, 0.3)
+    ch = 0.0 if ct else float(smax)
+    decay = 1.0 if ct else 0.93
+    c = _corr(n, s.conserved(), seed=17, dtype=dtype)
+    if flags & 16:
+        c[4] = (np.random.default_rng(18).random(n ** 3) < 0.1).astype(dtype)
+    U0 = s.conserved()
+    slots0 = [s.ct_slot(q) for q in range(21)] if ct else None
+    # the seam's inputs: copies of the state before the step
+    ins = _device_fields(M, list(U0))
+    outs = _device_fields(M, [np.zeros_like(U0[0])] * 9)
+    ctbuf = _device_fields(M, [np.concatenate(slots0)])[0] if ct else None
+    corr_seam = M.correction_buffer(s, c) if radiation else None
+    corr_state = M.correction_buffer(s, c) if radiation else None
+    ws = M.allocate_state((n,) * 3, 1.0 / n, dtype=dtype, recon="plm", riemann=riemann, mhd_solver=solver,
+                          workspace=True)
+    M.step_sources_into(ws, [b.ptr for b in ins], [b.ptr for b in outs], float(dt), ch, decay, 0.3,
+                        correction=corr_seam, flags=flags, integrator=integ,
+                        ct_ptr=None if ctbuf is None else ctbuf.ptr)
+    ws.synchronize()
+    if radiation:
+        kw = dict(track_density=bool(flags & 1), track_magnetic=bool(flags & 2), track_energy=bool(flags & 4),
+                  check_admissibility=bool(flags & 8), fallback=bool(flags & 16), robust_fallback=bool(flags & 64))
+        M.step_radiation(s, float(dt), ch, decay, 0.3, corr_state, integrator=integ, **kw)
+    else:
+        M.step_drag(s, float(dt), ch, decay, 0.3, integrator=integ)
+    got = np.stack([b.download(dtype, n ** 3) for b in outs])
+    assert np.array_equal(got, s.conserved())
+    if radiation:
+        assert np.array_equal(corr_seam.download(dtype, (9, n ** 3)), corr_state.download(dtype, (9, n ** 3)))
+    if ct:
+        slots = ctbuf.download(dtype, (21, n ** 3))
+        for q in range(21):  # faces, mass fluxes, face fluxes, cell and edge EMFs, remainders
+            assert np.array_equal(slots[q], s.ct_slot(q)), q
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sources_into_other_arrays_leaves_the_kept_cfl_speed_alone(dtype):
+    """A coupled step of other arrays through a state must not leave their CFL speed
+    standing in for the state's own (the fused epilogue writes it to the state's device
+    scalar)."""
+    import pmfmhd as M
+    n = 16
+    s, _ = _pair(n, smooth_state(n, seed=3), dtype, "plm", "hlld")
+    dt, sm = M.compute_dt(s, 0.3)
+    M.step(s, float(dt), ch=float(sm), integrator="cube")
+    _, kept = M.compute_dt(s, 0.3)
+    other = 2.0 * smooth_state(n, seed=4)
+    ins = _device_fields(M, list(other.astype(dtype)))
+    outs = _device_fields(M, [np.zeros(n ** 3, dtype)] * 9)
+    M.step_sources_into(s, [b.ptr for b in ins], [b.ptr for b in outs], 1e-3, 3.0, 1.0, 0.2, integrator="cube")
+    _, again = M.compute_dt(s, 0.3)
+    assert again == kept
