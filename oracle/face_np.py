@@ -217,6 +217,41 @@ def riemann(Lq, Rq, d, ch, g, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4, llf_pmin=0.0
     return f[ROT_FROM[d]]
 
 
+def hlld_branches(Lq, Rq, d, ch, g, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4, llf_pmin=0.0):
+    """Which part of the HLLD dispatch each face takes (glm_physics.jl:255-307, 320-327):
+    -1 where the LLF switch takes the face, else 0..3 for the S_L*>=0 / S_M>=0 / S_R*>=0 /
+    S_R* < 0 flux branch; and the mask of faces with a degenerate single-star state
+    (|denK| < 1e-12 on either side). Test instrumentation: no flux is computed."""
+    L = Lq[ROT_TO[d]].copy()
+    R = Rq[ROT_TO[d]].copy()
+    for q in (L, R):
+        q[0] = np.maximum(q[0], smallr)
+        q[4] = np.maximum(q[4], pfl)
+    bn, _ = glm_pair(L[5], R[5], L[8], R[8], ch)
+    dL, uL, pL, byL, bzL = L[0], L[1], L[4], L[6], L[7]
+    dR, uR, pR, byR, bzR = R[0], R[1], R[4], R[6], R[7]
+    ptL = pL + 0.5 * (bn * bn + byL * byL + bzL * bzL)
+    ptR = pR + 0.5 * (bn * bn + byR * byR + bzR * bzR)
+    cf = np.maximum(fast_speed(L, g, bn), fast_speed(R, g, bn))
+    SL = np.minimum(np.minimum(uL, uR) - cf, 0.0)
+    SR = np.maximum(np.maximum(uL, uR) + cf, 0.0)
+    with np.errstate(all="ignore"):
+        den = (SR - uR) * dR - (SL - uL) * dL
+        SM = ((SR - uR) * dR * uR - (SL - uL) * dL * uL - ptR + ptL) / den
+        dLs = np.maximum(dL * (SL - uL) / (SL - SM), 1e-12)
+        dRs = np.maximum(dR * (SR - uR) / (SR - SM), 1e-12)
+        SLs, SRs = SM - np.abs(bn) / np.sqrt(dLs), SM + np.abs(bn) / np.sqrt(dRs)
+        degen = (np.abs(dL * (SL - uL) * (SL - SM) - bn * bn) < 1e-12) | \
+                (np.abs(dR * (SR - uR) * (SR - SM) - bn * bn) < 1e-12)
+    br = np.where(SLs >= 0, 0, np.where(SM >= 0, 1, np.where(SRs >= 0, 2, 3)))
+    use = np.zeros(dL.shape, dtype=bool)
+    if llf_dmin > 0:
+        use |= np.minimum(dL, dR) < llf_dmin
+    if llf_pmin > 0:
+        use |= np.minimum(pL, pR) < llf_pmin
+    return np.where(use, -1, br), degen & ~use
+
+
 def step_periodic(U, dims, dtdx, ch, decay, g=5.0 / 3.0, smallr=1e-6, pfl=1e-7, llf_dmin=1e-4, llf_pmin=0.0,
                   recon="plm", solver="hlld", flux=None):
     """One GLM step of a periodic grid; U is (9, nc) conserved, i fastest. ``flux(L, R, d)``

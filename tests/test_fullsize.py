@@ -118,3 +118,59 @@ def test_species_transport_at_full_size(state256):
     d1, d2 = b1.download(np.float32, nc), b2.download(np.float32, nc)
     assert d1.min() >= 0 and d2.min() >= 0
     assert ((d1.astype(np.float64) + d2) / rho).max() <= 0.9 * (1 + 1e-6)
+
+
+@pytest.mark.timeout(900)
+def test_config3_512_fused_equals_per_cell_and_keeps_the_ledger():
+    """512^3, the bandwidth-roofline configuration (Mach-1.5 solenoidal velocity, lognormal
+    rho sigma 0.3, |m| <= 64, rms|B| 1): one fused step equals the per-cell kernel's to
+    2e-6 of each field's maximum; five fused steps keep every field finite, the density
+    positive, and the float64 ledger's mass and energy totals to 1e-9."""
+    n = 512
+    print("config3 512^3: building the initial state", flush=True)
+    U = P.config_state("c3_512", seed=1).astype(np.float32)
+    a = M.allocate_state((n,) * 3, 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    b = M.allocate_state((n,) * 3, 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    a.set_conserved(U)
+    b.set_conserved(U)
+    del U
+    t0 = M.conserved_totals(a)
+    dt, sm = M.compute_dt(a, 0.2)
+    M.step(a, float(dt), ch=float(sm), integrator="cube")
+    M.step(b, float(dt), ch=float(sm), integrator="ref")
+    for v in range(9):
+        x, y = a.download(v).astype(np.float64), b.download(v).astype(np.float64)
+        ref = np.abs(y).max() if v < 8 else np.abs(b.download(5)).max()
+        assert np.max(np.abs(x - y)) <= 2e-6 * ref, v
+    del b, x, y
+    for _ in range(4):
+        dt, sm = M.compute_dt(a, 0.2)
+        M.step(a, float(dt), ch=float(sm), integrator="cube")
+    t1 = M.conserved_totals(a)
+    for k in ("mass", "energy"):
+        assert abs(t1[k] - t0[k]) <= 1e-9 * abs(t0[k]), k
+    rho = a.download(0)
+    assert np.isfinite(rho).all() and rho.min() > 0
+    for v in range(1, 9):
+        assert np.isfinite(a.download(v)).all(), v
+
+
+def test_surrogate_step_at_full_size_without_gated_faces_is_the_plain_step():
+    """256^3 on the surrogate workload's inputs (|m| <= 8, rms|B| 0.3, v = 0): with the gate
+    closed (thr = 0) no face takes the network and the surrogate kernel's step equals the
+    plain fused step to float32 rounding, with the same CFL speed."""
+    from pmfmhd.surrogate import Surrogate, step_surrogate
+    U = P.config_state("c4_256", seed=1).astype(np.float32)
+    a, b = _state(U), _state(U)
+    net = Surrogate(b, seed=1)
+    dt, smax = M.compute_dt(a, 0.2)
+    M.step(a, float(dt), ch=float(smax), integrator="cube")
+    assert step_surrogate(b, net, float(dt), float(smax), thr=0.0, count=True) == 0
+    # the momenta start at zero, so they are measured on the momentum scale of the state,
+    # sqrt(max rho * max E), and psi on the field's
+    mscale = np.sqrt(np.abs(a.download(0)).max() * np.abs(a.download(4)).max())
+    for v in range(9):
+        x, y = a.download(v).astype(np.float64), b.download(v).astype(np.float64)
+        ref = np.abs(x).max() if v in (0, 4, 5, 6, 7) else (mscale if v < 4 else np.abs(a.download(5)).max())
+        assert np.max(np.abs(x - y)) <= 2e-6 * ref, v
+    assert M.max_wavespeed(a) == pytest.approx(M.max_wavespeed(b), rel=1e-6)

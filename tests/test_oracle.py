@@ -405,3 +405,203 @@ def test_llf_consistency_and_minmod_convergence():
     assert min(orders) > 1.6, orders
     # minmod is more diffusive than MonCen on the same wave (validation-cpu.log:8-12)
     assert L1[0] > 6.919e-4
+
+
+# ---------------------------------------------------------------------------
+# Closed-form pins of the dissipation speeds, the floors and the HLLD limits. Each
+# expected value is computed here from textbook formulas, not from the oracle's code.
+
+def _cf_textbook(rho, p, B, bn, g=G):
+    """Fast magnetosonic speed: c_f^2 = (a^2 + b^2 + sqrt((a^2 + b^2)^2 - 4 a^2 b_n^2)) / 2
+    with a^2 = g p / rho, b^2 = |B|^2 / rho, b_n^2 = bn^2 / rho."""
+    a2, b2, bn2 = g * p / rho, np.dot(B, B) / rho, bn * bn / rho
+    return np.sqrt(0.5 * (a2 + b2 + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bn2)))
+
+
+def _rest(rho, p, B, psi=0.0):
+    return np.array([rho, 0.0, 0.0, 0.0, p, B[0], B[1], B[2], psi])
+
+
+def _perm(q, d):
+    """The state whose direction-d frame is q's x frame: normal components go to axis d."""
+    out = q.copy()
+    for base in (1, 5):
+        comp = q[base:base + 3]
+        out[base:base + 3] = np.roll(comp, d)
+    return out
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("ch,ch_dominates", [(10.0, True), (0.05, False)])
+def test_llf_dissipation_of_a_resting_density_jump(d, ch, ch_dominates):
+    """glm_physics.jl:239-244. Two resting states with equal p and B and a density jump:
+    the central part of every flux is the same on both sides, so F = F(U) - smax/2 (U_R - U_L)
+    with smax = max(c_f,L, c_f,R, ch). Only rho differs, so F_rho = -smax (rho_R - rho_L)/2
+    (negative: mass moves from the dense side), F_E = 0, the normal momentum flux is the
+    total pressure minus Bn^2, F_psi = ch^2 Bn and F_Bn = psi* = 0."""
+    B = np.array([0.4, -0.3, 0.2])
+    p, rl, rr = 0.7, 1.0, 2.5
+    L, R = _perm(_rest(rl, p, B), d), _perm(_rest(rr, p, B), d)
+    Fx = O.riemann_batch(L[None], R[None], d, ch, riemann="llf")[0]
+    cf = max(_cf_textbook(rl, p, B, B[0]), _cf_textbook(rr, p, B, B[0]))
+    smax = max(cf, ch)
+    assert (smax == ch) == ch_dominates
+    n = [1 + d, 5 + d]  # normal momentum and normal field slots in the grid frame
+    t = [1 + (d + 1) % 3, 1 + (d + 2) % 3]
+    want = np.zeros(9)
+    want[0] = -0.5 * smax * (rr - rl)
+    want[n[0]] = p + 0.5 * B @ B - B[0] ** 2
+    want[t[0]], want[t[1]] = -B[0] * B[1], -B[0] * B[2]
+    want[8] = ch * ch * B[0]
+    np.testing.assert_allclose(Fx, want, rtol=1e-14, atol=1e-15)
+    assert Fx[0] < 0
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_hll_dissipation_of_a_resting_density_jump(d):
+    """glm_physics.jl:245-252: with S_L = -S_R = -max c_f, HLL gives F_rho = -c_f (rho_R - rho_L)/2
+    and does not take ch into its speeds."""
+    B = np.array([0.4, -0.3, 0.2])
+    p, rl, rr = 0.7, 1.0, 2.5
+    L, R = _perm(_rest(rl, p, B), d), _perm(_rest(rr, p, B), d)
+    F = O.riemann_batch(L[None], R[None], d, 30.0, riemann="hll")[0]
+    cf = max(_cf_textbook(rl, p, B, B[0]), _cf_textbook(rr, p, B, B[0]))
+    assert F[0] == pytest.approx(-0.5 * cf * (rr - rl), rel=1e-14)
+    assert F[4] == pytest.approx(0.0, abs=1e-15)
+
+
+def test_cons2prim_floors_density_before_velocity_and_kinetic_energy():
+    """glm_physics.jl:11-18 through the predictor of a uniform neighbourhood (no slopes, so
+    mh = cons2prim(prim2cons(q))): a density below smallr is raised to smallr first, so
+    v = m / smallr and p = p + (g-1) rho v^2 (1 - rho/smallr) / 2; a state whose internal
+    energy is negative gets p = pfl."""
+    smallr, pfl = 1e-6, 1e-7
+    rho, v, p = 2.5e-7, np.array([0.3, -0.2, 0.1]), 0.02
+    B = np.array([0.1, 0.05, -0.2])
+    q_low = np.concatenate([[rho], v, [p], B, [0.01]])
+    q_neg = np.array([1.2, 0.4, 0.0, -0.3, -0.5, 0.2, 0.1, 0.3, -0.02])
+    M = np.stack([np.repeat(q[None], 7, axis=0) for q in (q_low, q_neg)])
+    mh, _, _ = O.predict_batch(M, 0.2, recon="plm", smallr=smallr, pfl=pfl)
+    np.testing.assert_allclose(mh[0, 0], smallr, rtol=0)
+    np.testing.assert_allclose(mh[0, 1:4], rho * v / smallr, rtol=1e-14)
+    want_p = p + (G - 1) * 0.5 * rho * (v @ v) * (1 - rho / smallr)
+    np.testing.assert_allclose(mh[0, 4], want_p, rtol=1e-12)
+    np.testing.assert_allclose(mh[0, 5:9], q_low[5:9], rtol=1e-15)
+    assert mh[1, 4] == pfl
+    np.testing.assert_allclose(mh[1, [0, 1, 2, 3, 5, 6, 7, 8]], q_neg[[0, 1, 2, 3, 5, 6, 7, 8]], rtol=1e-15,
+                               atol=1e-17)
+
+
+@pytest.mark.parametrize("riemann", ["llf", "hll", "hlld"])
+def test_riemann_floors_both_face_states(riemann):
+    """glm_physics.jl:320-321: the face states are floored again before the solve. Two resting,
+    field-free states whose densities and pressures are both below the floors are the same
+    state after flooring, so every solver returns the physical flux of (smallr, 0, pfl):
+    no mass flux and a normal momentum flux of exactly pfl (unfloored it would be -1.5)."""
+    smallr, pfl = 1e-6, 1e-7
+    L = np.array([1e-9, 0, 0, 0, -1.0, 0, 0, 0, 0])
+    R = np.array([3e-7, 0, 0, 0, -2.0, 0, 0, 0, 0])
+    for d in range(3):
+        F = O.riemann_batch(L[None], R[None], d, 1.0, riemann=riemann, smallr=smallr, pfl=pfl, llf_dmin=0.0)[0]
+        want = np.zeros(9)
+        want[1 + d] = pfl
+        np.testing.assert_allclose(F, want, rtol=1e-14, atol=1e-20)  # pfl * c_f ~ 3e-8
+
+
+def _hllc_normal_field(L, R, g=G):
+    """Textbook HLLC (Toro 2009, sec. 10.4; Batten et al. 1997 star pressure) for the x
+    flux of two states whose field is purely normal (By = Bz = 0, equal Bn). Such a field
+    only adds Bn^2/2 to the pressure and to the energy and is carried unchanged, so the
+    MHD flux is the hydro HLLC flux of (rho, rho u, E - Bn^2/2) with the fast speed
+    max(a, |Bn|/sqrt(rho)), the momentum flux lowered by Bn^2/2, F_E the hydro one,
+    and no transverse fluxes."""
+    bn = L[5]
+    out = []
+    for q in (L, R):
+        rho, u, p = q[0], q[1], q[4]
+        e_h = p / (g - 1) + 0.5 * rho * u * u
+        cf = max(np.sqrt(g * p / rho), abs(bn) / np.sqrt(rho))
+        out.append((rho, u, p, e_h, cf))
+    (rl, ul, pl, el, cfl), (rr, ur, pr, er, cfr) = out
+    SL = min(min(ul, ur) - max(cfl, cfr), 0.0)
+    SR = max(max(ul, ur) + max(cfl, cfr), 0.0)
+    SM = (pr - pl + rl * ul * (SL - ul) - rr * ur * (SR - ur)) / (rl * (SL - ul) - rr * (SR - ur))
+    pstar = pl + rl * (SL - ul) * (SM - ul)
+
+    def flux(rho, u, p, e):
+        return np.array([rho * u, rho * u * u + p, (e + p) * u])
+
+    def star(rho, u, p, e, S):
+        f = (S - u) / (S - SM)
+        return np.array([rho * f, rho * f * SM, ((S - u) * e - p * u + pstar * SM) / (S - SM)])
+
+    if SM >= 0:
+        F = flux(rl, ul, pl, el) + SL * (star(rl, ul, pl, el, SL) - np.array([rl, rl * ul, el]))
+    else:
+        F = flux(rr, ur, pr, er) + SR * (star(rr, ur, pr, er, SR) - np.array([rr, rr * ur, er]))
+    out = np.zeros(9)
+    out[0], out[1], out[4] = F[0], F[1] - 0.5 * bn * bn, F[2]
+    return out
+
+
+def test_hlld_reduces_to_hllc_for_a_purely_normal_field():
+    """glm_physics.jl:255-307 in the limit By = Bz = 0: the rotational discontinuities
+    vanish, both star states coincide with the double-star ones and HLLD is HLLC. Random
+    jumps in rho, u and p, including low-beta states (g p << Bn^2, where c_f is the
+    Alfven speed and the fast and Alfven waves are degenerate)."""
+    rng = np.random.default_rng(41)
+    n = 400
+    L = np.zeros((n, 9))
+    R = np.zeros((n, 9))
+    for q in (L, R):
+        q[:, 0] = rng.uniform(0.2, 3.0, n)
+        q[:, 1] = rng.normal(0, 1.0, n)
+        q[:, 4] = rng.uniform(0.05, 2.0, n)
+    bn = rng.normal(0, 1.0, n)
+    bn[: n // 4] *= 20.0  # low beta
+    L[:, 5] = R[:, 5] = bn
+    F = O.riemann_batch(L, R, 0, 1.0, riemann="hlld", llf_dmin=0.0)
+    for f in range(n):
+        want = _hllc_normal_field(L[f], R[f])
+        want[8] = 1.0 * bn[f]  # F_psi = ch^2 Bn* (psi = 0, equal Bn)
+        scale = max(np.abs(want).max(), 1.0)
+        assert np.max(np.abs(F[f] - want)) < 1e-12 * scale, f
+
+
+def test_hlld_degenerate_star_is_the_physical_flux():
+    """glm_physics.jl:255-263: a resting state with a purely normal field and g p << Bn^2
+    makes the fast speed the Alfven speed, so denK = rho (S-u)(S-SM) - Bn^2 vanishes
+    analytically on both sides. The solve must take the degenerate branch (no 0/0) and, the
+    two sides being equal, return the physical flux: only the momentum flux p - Bn^2/2 and
+    F_psi = ch^2 Bn. Also at the density floor (rho = smallr, Bn scaled to keep the
+    Alfven speed), where denK is a difference of numbers of order 1e-6."""
+    for rho, bn, p in ((1.0, 1.0, 1e-4), (1e-6, 1e-3, 1e-12), (2.0, -0.7, 3e-5)):
+        q = np.array([rho, 0, 0, 0, p, bn, 0, 0, 0])
+        for d in range(3):
+            qd = _perm(q, d)
+            F = O.riemann_batch(qd[None], qd[None], d, 2.0, riemann="hlld", llf_dmin=0.0, smallr=1e-6,
+                                pfl=1e-13)[0]
+            assert np.all(np.isfinite(F))
+            want = np.zeros(9)
+            want[1 + d] = p - 0.5 * bn * bn
+            want[8] = 4.0 * bn
+            np.testing.assert_allclose(F, want, rtol=1e-12, atol=1e-12 * max(abs(p), bn * bn))
+
+
+def test_llf_switch_takes_low_density_and_low_pressure_faces():
+    """glm_physics.jl:324-327: with min rho < llf_dmin (or llf_pmin > 0 and min p < llf_pmin)
+    the HLLD dispatch returns the LLF flux. A resting contact (density jump, equal p, normal
+    field) is exact for HLLD (no mass flux) and diffused by LLF (-smax/2 drho)."""
+    B = np.array([0.3, 0.0, 0.0])
+    L, R = _rest(5e-5, 1.0, B), _rest(2e-4, 1.0, B)
+    ch = 0.5
+    on = O.riemann_batch(L[None], R[None], 0, ch, riemann="hlld", llf_dmin=1e-4)[0]
+    off = O.riemann_batch(L[None], R[None], 0, ch, riemann="hlld", llf_dmin=0.0)[0]
+    cf = max(_cf_textbook(5e-5, 1.0, B, B[0]), _cf_textbook(2e-4, 1.0, B, B[0]), ch)
+    assert on[0] == pytest.approx(-0.5 * cf * (2e-4 - 5e-5), rel=1e-13)
+    assert off[0] == pytest.approx(0.0, abs=1e-15)
+    L2, R2 = _rest(1.0, 1e-3, B), _rest(3.0, 1e-3, B)
+    Lp = O.riemann_batch(L2[None], R2[None], 0, ch, riemann="hlld", llf_dmin=0.0, llf_pmin=1e-2)[0]
+    cf = max(_cf_textbook(1.0, 1e-3, B, B[0]), _cf_textbook(3.0, 1e-3, B, B[0]), ch)
+    assert Lp[0] == pytest.approx(-0.5 * cf * 2.0, rel=1e-13)
+    assert O.riemann_batch(L2[None], R2[None], 0, ch, riemann="hlld", llf_dmin=0.0)[0][0] == pytest.approx(0, abs=1e-14)
