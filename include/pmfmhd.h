@@ -99,6 +99,18 @@ int pmf_step_into(pmf_state* s, const void* const* U, void* const* O, void* ct, 
                   double decay, int32_t integrator);
 /* max_wavespeed! (timestep.jl:17-23) of the caller's nine field arrays. */
 int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax);
+/* pmf_step_into without a state and without allocating. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc). p gives the grid and the
+ * physics as for pmf_create; the step runs on hip_stream (NULL = the null stream) and its
+ * kernels' workspace is a library-held device array of 2^18 + 64 doubles (2 MiB), one per
+ * device, kept until the process ends: grids whose fused step needs more than 2^18
+ * per-workgroup partial slots (about 1.6e9 cells at the built-in chunk) are refused with an
+ * error naming pmf_step_into. smax_dev (optional, a caller device double) receives the
+ * CFL speed of the output fields (max_wavespeed!, timestep.jl:17-23). Calls that overlap
+ * on one device share the workspace and race; calls on one stream are ordered. */
+int pmf_step_into_static(const pmf_params* p, const void* const* U, void* const* O, void* ct, double dt, double ch,
+                         double decay, int32_t integrator, void* hip_stream, double* smax_dev);
 /* evolve! (driver.jl:48-71) without a callback: CFL steps until t >= tfinal(1-1e-9),
  * ch = glm_ch_fac*smax, dt scaled by smax/max(smax,ch), the last step clipped. */
 int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, double glm_cr,

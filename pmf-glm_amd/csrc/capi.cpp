@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -740,6 +741,97 @@ int pmf_max_wavespeed_of(pmf_state* s, const void* const* U, double* smax) {
     hip_check(hipGetLastError(), "wavespeed launch");
     fetch(s, 0, 1);
     *smax = s->hval[0];
+  });
+}
+
+// ---- The allocation-free step --------------------------------------------------------
+// pmf_step_into for callers that may not allocate device memory (code under HIP graph
+// capture, serving loops, hosts that forbid it): no pmf_state, the caller's stream, and
+// the kernels' workspace (per-workgroup partial maxima of the CFL fold, then the device
+// scalars) in one file-scope device array. Every block writes its own partial slot before
+// the final fold reads it, so nothing in the array has to start at zero. Calls that
+// overlap on one device share the array and race.
+namespace {
+constexpr int64_t kStaticPartials = int64_t(1) << 18;  // doubles: per-workgroup partials
+constexpr int64_t kStaticScalars = 64;                 // doubles: [0] = CFL speed of the output
+constexpr int kStaticMaxDevices = 64;
+}  // namespace
+
+// 2 MiB + 512 B on each device that runs pmf_step_into_static, held until the process ends.
+__device__ __align__(256) double scratch[kStaticPartials + kStaticScalars];
+
+namespace {
+std::mutex g_static_mu;
+double* g_static_ptr[kStaticMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`, looked up once per device with that
+// device current. Host code uses only this address, never the array's name.
+double* static_workspace() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kStaticMaxDevices)
+    throw Error("pmf_step_into_static: device ordinal " + std::to_string(dev) + " beyond the " +
+                std::to_string(kStaticMaxDevices) + " the library keeps workspace addresses for");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw Error("hipGetSymbolAddress(scratch) returned a null address");
+    g_static_ptr[dev] = (double*)ptr;
+  }
+  return g_static_ptr[dev];
+}
+
+// Doubles of partials the step of pmf_step_into needs for this grid: the fused GLM kernel
+// writes one per workgroup at the built-in chunk; the whole-grid reductions (CT's
+// cell-field pass, the CFL reduction of the output that smax_dev asks for after a
+// per-cell GLM step) use kRedSlots per block for at most kRedBlocks blocks.
+int64_t static_partials_needed(const pmf_params& p, int integrator, bool want_smax) {
+  const int64_t nc = int64_t(p.nx) * p.ny * p.nz;
+  const Grid G{p.nx, p.ny, p.nz, {p.bc[0], p.bc[1], p.bc[2]}, nc};
+  if (p.solver == PMF_SOLVER_GLM && integrator == PMF_INTEGRATOR_CUBE) return fused_blocks(G, 0);
+  if (p.solver == PMF_SOLVER_CT || want_smax) return int64_t(kRedBlocks) * kRedSlots;
+  return 0;
+}
+}  // namespace
+
+int pmf_step_into_static(const pmf_params* p, const void* const* U, void* const* O, void* ct, double dt, double ch,
+                         double decay, int32_t integrator, void* hip_stream, double* smax_dev) {
+  return guard([&] {
+    if (!p) throw Error("null argument");
+    validate_params(*p);
+    if (!U || !O) throw Error("null field array");
+    const int64_t need = static_partials_needed(*p, integrator, smax_dev != nullptr);
+    if (need > kStaticPartials)
+      throw Error("pmf_step_into_static: a " + std::to_string(p->nx) + "x" + std::to_string(p->ny) + "x" +
+                  std::to_string(p->nz) + " grid needs " + std::to_string(need) +
+                  " partial slots, more than the static workspace's " + std::to_string(kStaticPartials) +
+                  "; use pmf_step_into");
+    double* w = static_workspace();
+    pmf_state s;  // parameters, the caller's stream and the static workspace; owns nothing
+    s.p = *p;
+    s.nc = int64_t(p->nx) * p->ny * p->nz;
+    s.G = Grid{p->nx, p->ny, p->nz, {p->bc[0], p->bc[1], p->bc[2]}, s.nc};
+    s.esz = p->precision == PMF_F64 ? 8 : 4;
+    s.st = (hipStream_t)hip_stream;
+    s.red = w;
+    s.dval = w + kStaticPartials;
+    const bool folded = p->solver == PMF_SOLVER_CT || integrator == PMF_INTEGRATOR_CUBE;
+    if (p->precision == PMF_F64) {
+      step_fields<double>(&s, cfields_from<double>(U), fields_from<double>(O), (double*)ct, dt, ch, decay, integrator);
+      if (smax_dev && !folded)
+        launch_wavespeed<double>(cfields_from<double>((const void* const*)O), s.G, make_params<double>(&s, 0, 0, 1),
+                                 s.red, s.dval, s.st);
+    } else {
+      step_fields<float>(&s, cfields_from<float>(U), fields_from<float>(O), (float*)ct, dt, ch, decay, integrator);
+      if (smax_dev && !folded)
+        launch_wavespeed<float>(cfields_from<float>((const void* const*)O), s.G, make_params<float>(&s, 0, 0, 1),
+                                s.red, s.dval, s.st);
+    }
+    hip_check(hipGetLastError(), "static step launch");
+    if (smax_dev)
+      hip_check(hipMemcpyAsync(smax_dev, s.dval, sizeof(double), hipMemcpyDeviceToDevice, s.st),
+                "hipMemcpyAsync(smax)");
   });
 }
 
