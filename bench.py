@@ -115,8 +115,42 @@ def error_report(M, P, n, steps):
     ref, seq = fp64_sequence(M, U, n, steps)
     got, _, _ = replay(M, U, n, seq)
     return {"workload": f"plain GLM-HLLD {n}^3, seed 1, {steps} CFL-0.2 steps", "steps": steps,
-            "reference": "float64 GPU fused kernel (float64 oracle parity 1e-12), dt sequence replayed in float32",
+            "reference": "float64 GPU fused kernel (equal to the float64 CPU oracle to 1e-12 in the parity tests and "
+                         "in fp64_oracle_check), its dt sequence replayed in float32",
             "rel_l2_fp32_vs_fp64": rel_l2(got, ref)}
+
+
+def oracle_error_report(M, P, n, steps):
+    """The float32 GPU run on the benchmark recipe at n^3 (its own CFL dt sequence, the
+    fused step) against the float64 CPU oracle replaying that (dt, ch) sequence from the
+    same initial state: the whole-run relative L2 per variable. The oracle is the C++
+    restatement of the reference's :ref kernel, OpenMP over the host cores."""
+    import numpy as np
+    import oracle as O
+    U = P.config_state("c2_256", seed=1, n=n)
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(U)
+    seq = []
+    for _ in range(steps):
+        dt, sm = M.compute_dt(s, 0.2)
+        seq.append((float(dt), float(sm)))
+        M.step(s, float(dt), ch=float(sm), integrator="cube")
+    got = s.conserved()
+    del s
+    o = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
+    o.set_conserved(U)
+    t0 = time.perf_counter()
+    for k, (dt, ch) in enumerate(seq):
+        o.step(dt, ch=ch)
+        if (k + 1) % 50 == 0:
+            log(f"bench: float64 CPU oracle replay, step {k + 1}/{steps} ({time.perf_counter() - t0:.0f} s)")
+    el = time.perf_counter() - t0
+    return {"workload": f"plain GLM-HLLD {n}^3 (the benchmark recipe with |m| <= N/8), seed 1, {steps} CFL-0.2 "
+                        f"steps",
+            "steps": steps,
+            "reference": "float64 CPU oracle (C++ restatement of the reference :ref kernel), replaying the float32 "
+                         "GPU run's (dt, ch) sequence",
+            "rel_l2_fp32_vs_fp64": rel_l2(got, o.U), "oracle_seconds": el, "oracle_cores": O.num_threads()}
 
 
 def fp64_oracle_check(M, P, n, steps):
@@ -167,6 +201,30 @@ def surrogate_leg(M, P, n, steps, weights, thr):
             "gate_sweep": sweep,
             "fastest_meeting_rel_l2_1e-4": fastest_meeting(cells * steps / t_sur, e_sur["all_but_psi"], thr, sweep,
                                                            cells * steps / t_plain)}
+
+
+def surrogate_profile(M, P, n, steps, warmup, weights, thr):
+    """Surrogate-assisted steps only (the profiled kernel), on the surrogate workload's
+    inputs, each step with its own CFL dt; the network-solved face fraction of the last."""
+    import numpy as np
+    from pmfmhd.surrogate import Surrogate, step_surrogate
+    s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
+    s.set_conserved(P.config_state("c4_256", seed=1, n=n))
+    net = Surrogate(s, seed=0)
+    net.set_weights(np.load(weights))
+    for _ in range(warmup):
+        dt, sm = M.compute_dt(s, 0.2)
+        step_surrogate(s, net, float(dt), float(sm), thr=thr)
+    s.synchronize()
+    t0 = time.perf_counter()
+    nsm = 0
+    for k in range(steps):
+        dt, sm = M.compute_dt(s, 0.2)
+        nsm = step_surrogate(s, net, float(dt), float(sm), thr=thr, count=k == steps - 1)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": f"surrogate-assisted step only, {n}^3 surrogate inputs, gate {thr}", "steps": steps,
+            "ms_per_step": el / max(steps, 1) * 1e3, "network_face_fraction_last": (nsm or 0) / (3 * n ** 3)}
 
 
 def fastest_meeting(value, err, thr, sweep, plain_value):
@@ -404,13 +462,20 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in z-chunk depth")
     ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
-    ap.add_argument("--fp64-oracle-steps", type=int, default=10,
-                    help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~4 s per step)")
+    ap.add_argument("--fp64-oracle-steps", type=int, default=5,
+                    help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~3 s per step)")
+    ap.add_argument("--oracle-error-n", type=int, default=128,
+                    help="grid of the float32-GPU-vs-float64-CPU-oracle error report (0 = skip)")
+    ap.add_argument("--oracle-error-steps", type=int, default=200,
+                    help="steps of that report (about 75 s of oracle work at 128^3 on 16 cores)")
     ap.add_argument("--ct-steps", type=int, default=200, help="steps of the CT leg (0 = skip)")
     ap.add_argument("--coupled-steps", type=int, default=20,
                     help="steps of each coupled (drag, radiation) leg (0 = skip)")
     ap.add_argument("--refbench", action=argparse.BooleanOptionalAction, default=True,
                     help="the reference's own benchmark workload (scripts/benchmark.jl), GLM at N=128 and CT at N=96")
+    ap.add_argument("--surrogate-profile", type=float, default=None, metavar="THR",
+                    help="only run --steps surrogate-assisted steps at gate THR on the 256^3 surrogate inputs "
+                         "(after --warmup), for a profiler; prints one JSON line of its own")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, copy the 9 state fields device-to-device once (a plain copy of known "
                          "size that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -426,6 +491,10 @@ def main():
     from pmfmhd import problems as P
 
     n = args.n
+    if args.surrogate_profile is not None:
+        print(json.dumps(surrogate_profile(M, P, n, args.steps, args.warmup, args.surrogate_weights,
+                                           args.surrogate_profile)), flush=True)
+        return
     log(f"bench: building the {n}^3 initial state")
     t0 = time.perf_counter()
     U = P.config_state("c2_256", seed=1, n=n)
@@ -503,7 +572,11 @@ def main():
     if not args.no_ensemble:
         log("bench: ensemble leg")
         ens = ensemble_leg(P)
-    c3 = ct = orc = None
+    c3 = ct = orc = oerr = None
+    if args.oracle_error_n > 0 and args.oracle_error_steps > 0 and integ == "cube":
+        log(f"bench: float32 GPU vs float64 CPU oracle error report ({args.oracle_error_n}^3, "
+            f"{args.oracle_error_steps} steps)")
+        oerr = oracle_error_report(M, P, args.oracle_error_n, args.oracle_error_steps)
     if args.fp64_oracle_steps > 0 and integ == "cube":
         log(f"bench: float64 GPU vs CPU oracle ({args.fp64_oracle_steps} steps)")
         orc = fp64_oracle_check(M, P, n, args.fp64_oracle_steps)
@@ -548,7 +621,7 @@ def main():
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_launch": BYTES_PER_CELL * cells},
         "valu_roofline": valu_from_profile(n, integ, avg_kernel_s),
         "cpu_baseline": cpu,
-        "error_report": errors,
+        "error_report": dict(oerr or {}, float64_gpu_reference_at_benchmark_size=errors),
         "surrogate": sur,
         "ensemble": ens,
         "c3_512": c3,
