@@ -221,6 +221,18 @@ int pmf_step_sources_into(pmf_state* s, const void* const* U, void* const* O, vo
                           double ch, double decay, double drag_impulse, const double* target_velocity,
                           int32_t flags, int32_t integrator);
 
+/* ---- State handoff files ----------------------------------------------------------
+ * The production runner's raw handoff (src/production/runner.jl:3814-3883): in dir, the
+ * files mhd_U_1.f32 ... mhd_U_9.f32 hold the nine conserved variables as nx*ny*nz raw
+ * little-endian float32 values each, x fastest. Import: every file must hold exactly
+ * that many bytes ("bad raw Float32 size for <path>: got <n> bytes, expected <m>"; a
+ * missing file reads as 0 bytes) and the minimum density and energy must be positive
+ * and finite ("imported MHD state has invalid minimum density <x>" / "... energy <x>");
+ * all checks run before the state is touched. float64 states convert on the way in and
+ * out. Export: each file written through <path>.tmp and a rename. */
+int pmf_import_mhd_raw(pmf_state* s, const char* dir);
+int pmf_export_mhd_raw(pmf_state* s, const char* dir);
+
 /* ---- Passive species (passive_transport.jl) --------------------------------------
  * conserved_species_to_fraction(s)!: species densities to bounded mass fractions in
  * place, using rho (NULL: the state's density); with two species the second bounds the

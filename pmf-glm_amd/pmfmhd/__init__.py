@@ -11,6 +11,7 @@ fallback, and a missing library raises.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -355,6 +356,20 @@ def step_sources_into(ws: MHDState, U_ptrs, O_ptrs, dt, ch, decay, drag_impulse,
     ct = None if ct_ptr is None else ctypes.c_void_p(int(ct_ptr))
     check(lib().pmf_step_sources_into(ws.handle, _ptrs(U_ptrs), _ptrs(O_ptrs), ct, corr, float(dt), float(ch),
                                       float(decay), float(drag_impulse), v0, int(flags), _integ(integrator)))
+
+
+def import_mhd_raw(s: MHDState, directory):
+    """_import_initial_mhd_state! (runner.jl:3868-3883): the state's nine fields from the
+    raw float32 handoff files mhd_U_1.f32 ... mhd_U_9.f32 of a directory."""
+    check(lib().pmf_import_mhd_raw(s.handle, os.fsencode(directory)))
+    return s
+
+
+def export_mhd_raw(s: MHDState, directory):
+    """The nine fields as the runner's raw float32 handoff files (runner.jl:3814-3825)."""
+    os.makedirs(directory, exist_ok=True)
+    check(lib().pmf_export_mhd_raw(s.handle, os.fsencode(directory)))
+    return s
 
 
 def set_chunk(s: MHDState, zc: int):

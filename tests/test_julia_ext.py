@@ -28,6 +28,7 @@ C_TO_JULIA = {
     "pmf_surrogate**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"},
     "pmf_ensemble**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"},
     "const int32_t*": {"Ptr{Int32}", "Ref{Int32}"},
+    "const char*": {"Cstring", "Ptr{UInt8}"},
     "const pmf_params*": {"Ref{PmfParams}", "Ptr{PmfParams}"},
     "pmf_params*": {"Ref{PmfParams}", "Ptr{PmfParams}"},
     "double": {"Cdouble", "Float64"},
