@@ -514,28 +514,50 @@ def main():
 
     tuned = None
     if integ == "cube" and not args.no_autotune:
+        # the tuner writes only the scratch fields: the state is still U, and the warm-up
+        # follows on a GPU that has just been busy (no idle upload in between)
         zc, ms = M.autotune(s, reps=5)
         tuned = {"zc": zc, "ms": ms}
         log(f"bench: autotuned z chunk {zc} planes ({ms:.3f} ms/step)")
-        s.set_conserved(U)
 
     def one_step():
         dt, sm = M.compute_dt(s, 0.2)
         M.step(s, float(dt), ch=float(sm), glm_cr=M.GLM_CR, integrator=integ)
 
+    # The fused path steps with pmf_advance: every step's dt, ch and psi decay are computed
+    # on the device from the speed the previous step folded (evolve!'s rule with
+    # glm_ch_fac 1), bit for bit the compute_dt + step pair, without a host round trip.
+    device_dt = integ == "cube"
     log(f"bench: warmup {args.warmup} steps ({integ})")
-    for _ in range(args.warmup):
-        one_step()
+    if device_dt:
+        M.advance(s, args.warmup, cfl=0.2)
+    else:
+        for _ in range(args.warmup):
+            one_step()
     s.synchronize()
     s.timing(True)
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step()
+    if device_dt:
+        M.advance(s, args.steps, cfl=0.2)
+    else:
+        for _ in range(args.steps):
+            one_step()
     s.synchronize()
     el = time.perf_counter() - t0
     kms, nl = s.timing_read()
     s.timing(False)
+    host_loop = None
+    if device_dt:
+        # the same steps driven from the host (one compute_dt round trip per step)
+        hn = min(args.steps, 20)
+        s.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(hn):
+            one_step()
+        s.synchronize()
+        el1 = time.perf_counter() - t1
+        host_loop = {"steps": hn, "ms_per_step": el1 / hn * 1e3, "value": n ** 3 * hn / el1}
     cells = n ** 3
     value = cells * args.steps / el
     avg_kernel_s = kms / 1e3 / max(nl, 1)
@@ -615,7 +637,9 @@ def main():
                 "rho = 1 + 0.01 N(0,1), p = 1",
         "config": {"workload": f"plain GLM-HLLD FP32 {n}^3, PLM minmod, CFL 0.2, gamma 5/3, periodic",
                    "n": n, "recon": "plm_minmod", "riemann": "hlld", "cfl": 0.2, "integrator": integ,
-                   "parallelism": "single GPU", "finite": bool(finite), "autotuned_z_chunk": tuned},
+                   "parallelism": "single GPU", "finite": bool(finite), "autotuned_z_chunk": tuned,
+                   "dt": "on the device (pmf_advance)" if device_dt else "on the host (compute_dt per step)"},
+        "host_driven_steps": host_loop,
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n, integ),
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_launch": BYTES_PER_CELL * cells},

@@ -116,6 +116,16 @@ int pmf_step_into_static(const pmf_params* p, const void* const* U, void* const*
 int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, double glm_cr,
                int32_t integrator, double* t_out, int64_t* nsteps_out);
 
+/* nsteps steps of evolve!'s rule (driver.jl:48-71: dt = cfl*dx/smax, ch =
+ * glm_ch_fac*smax, dt scaled by smax/max(smax,ch), psi decay exp(-glm_cr*ch*dt/dx)) on the
+ * fused kernel, with each step's dt computed on the device from the speed the previous
+ * step folded: the host enqueues every step without waiting (GLM states). Each step is
+ * the pmf_compute_dt + pmf_step pair bit for bit. t_out (optional) receives the elapsed
+ * time and dt_out the last step's dt; the call waits for the device once, at the end.
+ * No counterpart in the reference, whose driver computes every dt on the host. */
+int pmf_advance(pmf_state* s, int64_t nsteps, double cfl, double glm_ch_fac, double glm_cr, double* t_out,
+                double* dt_out);
+
 /* Diagnostics. max_divb (diagnostics.jl:6-36): BC-aware centred |div B| maximum.
  * pmf_divb_norm_fp64: the same divergence's float64 L2 norm sqrt(mean(d^2)) and max,
  * with compensated summation. conserved_totals (diagnostics.jl:44-55): float64

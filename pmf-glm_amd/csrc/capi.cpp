@@ -490,6 +490,64 @@ int pmf_evolve(pmf_state* s, double tfinal, double cfl, double glm_ch_fac, doubl
   });
 }
 
+extern "C++" {
+namespace {
+// Device scalar block (64 doubles): [0] CFL speed, [1..3] div B, [4..11] totals,
+// [12] face div B, [16..47] the device-resident step scalars of pmf_advance, [48..49]
+// its clock (elapsed time, last dt).
+constexpr int kAdvParams = 16, kAdvClock = 48;
+static_assert(sizeof(StepParams<double>) <= (kAdvClock - kAdvParams) * sizeof(double), "step scalars fit");
+
+template <class T>
+void advance_steps(pmf_state* s, int64_t nsteps, double cfl, double fac, double cr) {
+  auto* Pd = reinterpret_cast<StepParams<T>*>(s->dval + kAdvParams);
+  double* clock = s->dval + kAdvClock;
+  const StepParams<T> base = make_params<T>(s, 0, 0, 1);
+  if (!s->smax_valid) {
+    launch_wavespeed<T>(cfields_at<T>(s->U, s->nc), s->G, base, s->red, s->dval, s->st);
+    hip_check(hipGetLastError(), "wavespeed launch");
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    launch_step_params<T>(s->dval, base, Pd, clock, cfl, fac, cr, s->p.dx, k == 0, s->st);
+    mark(s);
+    const bool done = launch_step_fused<T>(cfields_at<T>(s->U, s->nc), fields_at<T>(s->S, s->nc), s->G, base, s->red,
+                                           s->dval, s->st, s->zc, nullptr, Pd);
+    mark(s);
+    if (!done) throw Error("pmf_advance: fused kernel unavailable for this configuration");
+    hip_check(hipGetLastError(), "step launch");
+    std::swap(s->U, s->S);
+  }
+  s->smax_valid = true;  // the last step folded its new state's speed into dval[0]
+}
+}  // namespace
+}  // extern "C++"
+
+int pmf_advance(pmf_state* s, int64_t nsteps, double cfl, double glm_ch_fac, double glm_cr, double* t_out,
+                double* dt_out) {
+  return guard([&] {
+    check_fields(s);
+    if (s->p.solver != PMF_SOLVER_GLM)
+      throw Error("pmf_advance runs the fused GLM step; constrained-transport states step with pmf_step");
+    if (nsteps < 0) throw Error("the number of steps must be non-negative");
+    if (!(glm_ch_fac > 0)) throw Error("glm_ch_fac must be positive");
+    if (!(glm_cr >= 0)) throw Error("glm_cr must be non-negative");
+    if (!(cfl > 0)) throw Error("cfl must be positive");
+    if (!fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann))
+      throw Error("step_cube!: the fused kernel needs every axis at least one cell and reflecting axes two (got " +
+                  std::to_string(s->p.nx) + "x" + std::to_string(s->p.ny) + "x" + std::to_string(s->p.nz) + ")");
+    if (nsteps == 0) {
+      if (t_out) *t_out = 0.0;
+      if (dt_out) *dt_out = 0.0;
+      return;
+    }
+    if (s->p.precision == PMF_F64) advance_steps<double>(s, nsteps, cfl, glm_ch_fac, glm_cr);
+    else advance_steps<float>(s, nsteps, cfl, glm_ch_fac, glm_cr);
+    fetch(s, kAdvClock, 2);
+    if (t_out) *t_out = s->hval[kAdvClock];
+    if (dt_out) *dt_out = s->hval[kAdvClock + 1];
+  });
+}
+
 int pmf_set_chunk(pmf_state* s, int32_t zc) {
   return guard([&] {
     check_state(s);

@@ -22,9 +22,18 @@ void launch_step_ref(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, cons
 // shape is not supported by the fused kernel.
 // zc: planes per z chunk (0 = the built-in choice); src: drag / radiation source terms
 // (nullptr for the plain step).
+// Pd: device-resident step scalars that replace P's (dt, ch, decay) in the kernel (P still
+// selects the reconstruction and Riemann solver); nullptr = P as passed.
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
+                       double* dsmax, hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr,
+                       const StepParams<T>* Pd = nullptr);
+// The device-side CFL stage of pmf_advance (evolve!'s rule, driver.jl:48-71, evaluated as
+// the host does it): from the speed in dsmax[0], writes the step scalars to Pd and adds
+// the step to clock[0] (reset: sets it), clock[1] = the step's dt.
+template <class T>
+void launch_step_params(const double* dsmax, const StepParams<T>& base, StepParams<T>* Pd, double* clock, double cfl,
+                        double glm_ch_fac, double glm_cr, double dx, int reset, hipStream_t st);
 bool fused_supported(const Grid& G, int precision, int recon, int rsolve);
 bool all_periodic(const Grid& G);
 bool surrogate_grid(const Grid& G);

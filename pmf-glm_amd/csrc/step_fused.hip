@@ -568,8 +568,10 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         }
       }
     } else if (xy) {
-      T Fx[9], Fy[9];
+      // Each face's left-state slot is read and then overwritten with its flux by the one
+      // thread that solves the face, so the fluxes go back to LDS without a barrier.
       {
+      T Fx[9], Fy[9];
       // masked fallback: LLF on every face touching a flagged cell (integrator_cube.jl:68-80)
       if (xsolver) {
         T L[9];
@@ -591,6 +593,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         } else {
           face_flux<0, false, T, RS>(L, Rx, Fx, P);
         }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix - 1, iy)] = Fx[v];
       }
       if (ysolver) {
         T L[9];
@@ -612,16 +616,9 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         } else {
           face_flux<1, false, T, RS>(L, Ry, Fy, P);
         }
-      }
-      }
-      __syncthreads();
-      if (xsolver) {
-#pragma unroll
-        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix - 1, iy)] = Fx[v];
-      }
-      if (ysolver) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy - 1)] = Fy[v];
+      }
       }
       __syncthreads();
       if (owned) {
@@ -698,18 +695,24 @@ __device__ __forceinline__ void step_fused_body(const Fields<const T>& U, const 
   }
 }
 
+// Pd: when not null, the step's scalars come from device memory (written there by the
+// device-side CFL stage of pmf_advance) instead of the kernel argument P.
 template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW, bool BCG>
 __global__ __launch_bounds__(FTHREADS) void k_step_fused(Fields<const T> U, Fields<T> O, FusedGeom g,
-                                                         StepParams<T> P, double* __restrict__ red,
+                                                         StepParams<T> P, const StepParams<T>* __restrict__ Pd,
+                                                         double* __restrict__ red,
                                                          const unsigned char* __restrict__ simg, float thr,
                                                          unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+  if (Pd) P = *Pd;
   step_fused_body<T, RECON, RS, SURR, SRC, NARROW, BCG>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
 template <class T, int RECON, int RS, bool SURR, bool SRC, bool NARROW, bool BCG>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused3(
-    Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, double* __restrict__ red,
-    const unsigned char* __restrict__ simg, float thr, unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+    Fields<const T> U, Fields<T> O, FusedGeom g, StepParams<T> P, const StepParams<T>* __restrict__ Pd,
+    double* __restrict__ red, const unsigned char* __restrict__ simg, float thr,
+    unsigned long long* __restrict__ nsmooth, SrcParams<T> src) {
+  if (Pd) P = *Pd;
   step_fused_body<T, RECON, RS, SURR, SRC, NARROW, BCG>(U, O, g, P, red, simg, thr, nsmooth, src);
 }
 
@@ -726,24 +729,26 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2))) v
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool BCG = false>
 void launch_fused_variant(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
                           const StepParams<T>& P, double* red, const unsigned char* simg, float thr,
-                          unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{}) {
+                          unsigned long long* nsmooth, const SrcParams<T>& src = SrcParams<T>{},
+                          const StepParams<T>* Pd = nullptr) {
   if constexpr (fused_three_waves<T, RECON, RS, SURR, SRC>())
     hipLaunchKernelGGL((k_step_fused3<T, RECON, RS, SURR, SRC, NARROW, BCG>), grd, dim3(FTHREADS), 0, st, U, O, g, P,
-                       red, simg, thr, nsmooth, src);
+                       Pd, red, simg, thr, nsmooth, src);
   else if constexpr (SURR)
     hipLaunchKernelGGL((k_step_fused2<T, RECON, RS, SURR, SRC, NARROW>), grd, dim3(FTHREADS), 0, st, U, O, g, P, red,
                        simg, thr, nsmooth, src);
   else
     hipLaunchKernelGGL((k_step_fused<T, RECON, RS, SURR, SRC, NARROW, BCG>), grd, dim3(FTHREADS), 0, st, U, O, g, P,
-                       red, simg, thr, nsmooth, src);
+                       Pd, red, simg, thr, nsmooth, src);
 }
 
 // The plain float step with 32-bit offsets (double steps never take this path).
 template <class T, int RECON, int RS>
 void launch_plain_narrow(dim3 grd, hipStream_t st, const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
-                         const StepParams<T>& P, double* red) {
+                         const StepParams<T>& P, double* red, const StepParams<T>* Pd) {
   if constexpr (sizeof(T) == 4)
-    launch_fused_variant<T, RECON, RS, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr);
+    launch_fused_variant<T, RECON, RS, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr,
+                                                           SrcParams<T>{}, Pd);
 }
 
 // Fields of nc cells addressable with 32-bit byte offsets. PMFMHD_WIDE_OFFSETS=1 in the
@@ -765,6 +770,55 @@ __global__ __launch_bounds__(256) void k_fold_max_flat(const double* __restrict_
   __syncthreads();
   if (threadIdx.x == 0) out[0] = nanmax(nanmax(sh[0], sh[1]), nanmax(sh[2], sh[3]));
 }
+
+// The device-side CFL stage of pmf_advance: evolve!'s step rule (driver.jl:48-71) from
+// the speed the previous step folded into dsmax[0], in the same operations and
+// precisions as the host path (pmf_compute_dt, pmf_evolve, decay_of, make_params), so
+// that a step with device-resident scalars is the host-driven step bit for bit.
+template <class T>
+__global__ __launch_bounds__(64) void k_step_params(const double* __restrict__ dsmax, StepParams<T> base,
+                                                    StepParams<T>* __restrict__ Pd, double* __restrict__ clock,
+                                                    double cfl, double fac, double cr, double dx, int reset) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double sm = dsmax[0];
+  double dtf, ch, decay;
+  if constexpr (sizeof(T) == 8) {
+    const double dt = cfl * dx / (sm < __DBL_EPSILON__ ? __DBL_EPSILON__ : sm);
+    ch = fac * sm;
+    dtf = dt * sm / (sm < ch ? ch : sm);
+    decay = exp(-cr * ch * dtf / dx);
+  } else {
+    const float smf = float(sm), dxf = float(dx);
+    const double dt = double(float(cfl) * dxf / (smf < __FLT_EPSILON__ ? __FLT_EPSILON__ : smf));
+    const float chf = float(fac) * smf;
+    ch = chf;
+    dtf = dt * double(smf) / double(smf < chf ? chf : smf);
+    const float v = -float(cr) * chf * float(dtf) / dxf;
+    decay = double(float(exp(double(v))));
+  }
+  StepParams<T> P = base;
+  P.ch = T(ch);
+  P.ich = ch != 0.0 ? T(1) / T(ch) : T(0);
+  P.ch2 = T(ch) * T(ch);
+  P.dtdx = T(dtf) / T(dx);
+  P.hdtdx = T(0.5) * P.dtdx;
+  P.decay = T(decay);
+  *Pd = P;
+  clock[0] = reset ? dtf : clock[0] + dtf;
+  clock[1] = dtf;
+}
+
+template <class T>
+void launch_step_params(const double* dsmax, const StepParams<T>& base, StepParams<T>* Pd, double* clock, double cfl,
+                        double glm_ch_fac, double glm_cr, double dx, int reset, hipStream_t st) {
+  hipLaunchKernelGGL((k_step_params<T>), dim3(1), dim3(64), 0, st, dsmax, base, Pd, clock, cfl, glm_ch_fac, glm_cr, dx,
+                     reset);
+}
+template void launch_step_params<float>(const double*, const StepParams<float>&, StepParams<float>*, double*, double,
+                                        double, double, double, int, hipStream_t);
+template void launch_step_params<double>(const double*, const StepParams<double>&, StepParams<double>*, double*,
+                                         double, double, double, double, int, hipStream_t);
 
 static FusedGeom fused_geom(const Grid& G, int zc) {
   FusedGeom g;
@@ -839,7 +893,7 @@ int64_t fused_blocks(const Grid& G, int zc) {
 
 template <class T>
 bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const StepParams<T>& P, double* red,
-                       double* dsmax, hipStream_t st, int zc, const SrcParams<T>* src) {
+                       double* dsmax, hipStream_t st, int zc, const SrcParams<T>* src, const StepParams<T>* Pd) {
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   unsigned nb = (unsigned)((int64_t)g.ntx * g.nty * g.nchunks);
@@ -849,10 +903,11 @@ bool launch_step_fused(Fields<const T> U, Fields<T> O, const Grid& G, const Step
 #define PMF_FUSED(R, S)                                                                                             \
   if (gen && src)                                                                                                  \
     launch_fused_variant<T, R, S, false, true, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src); \
-  else if (gen) launch_fused_variant<T, R, S, false, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr); \
+  else if (gen) launch_fused_variant<T, R, S, false, false, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f,     \
+                                                                         nullptr, SrcParams<T>{}, Pd);              \
   else if (src) launch_fused_variant<T, R, S, false, true>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, *src);   \
-  else if (narrow) launch_plain_narrow<T, R, S>(grd, st, U, O, g, P, red);                                          \
-  else launch_fused_variant<T, R, S, false, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr)
+  else if (narrow) launch_plain_narrow<T, R, S>(grd, st, U, O, g, P, red, Pd);                                      \
+  else launch_fused_variant<T, R, S, false, false>(grd, st, U, O, g, P, red, nullptr, 0.f, nullptr, SrcParams<T>{}, Pd)
 #define PMF_FUSED_RS(R)                                  \
   if (P.rsolve == RS_HLL) PMF_FUSED(R, RS_HLL);          \
   else if (P.rsolve == RS_LLF) PMF_FUSED(R, RS_LLF);     \
@@ -1150,8 +1205,10 @@ template bool launch_step_fused_ct<double>(Fields<const double>, Fields<double>,
                                            const StepParams<double>&, hipStream_t, int, const SrcParams<double>*);
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
-                                       double*, double*, hipStream_t, int, const SrcParams<float>*);
+                                       double*, double*, hipStream_t, int, const SrcParams<float>*,
+                                       const StepParams<float>*);
 template bool launch_step_fused<double>(Fields<const double>, Fields<double>, const Grid&, const StepParams<double>&,
-                                        double*, double*, hipStream_t, int, const SrcParams<double>*);
+                                        double*, double*, hipStream_t, int, const SrcParams<double>*,
+                                        const StepParams<double>*);
 
 }  // namespace pmf

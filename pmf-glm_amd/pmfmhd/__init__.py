@@ -428,6 +428,16 @@ def evolve(s: MHDState, tfinal, cfl=0.4, glm_ch_fac=1.0, glm_cr=GLM_CR, integrat
     return t, n
 
 
+def advance(s: MHDState, nsteps: int, cfl=0.4, glm_ch_fac=1.0, glm_cr=GLM_CR):
+    """nsteps CFL steps of evolve!'s rule (driver.jl:48-71) on the fused kernel with every
+    dt computed on the device (no host round trip per step); equal to nsteps
+    compute_dt + step(..., ch = glm_ch_fac * smax) pairs. Returns (elapsed t, last dt)."""
+    t, dt = ctypes.c_double(), ctypes.c_double()
+    check(lib().pmf_advance(s.handle, int(nsteps), float(cfl), float(glm_ch_fac), float(glm_cr), ctypes.byref(t),
+                            ctypes.byref(dt)))
+    return t.value, dt.value
+
+
 def max_divb(s: MHDState) -> float:
     v = ctypes.c_double()
     check(lib().pmf_max_divb(s.handle, ctypes.byref(v)))

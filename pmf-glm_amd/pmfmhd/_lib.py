@@ -38,6 +38,7 @@ def lib():
             "pmf_step_into": [P, P, P, P, d, d, d, i32],
             "pmf_step_into_static": [ctypes.POINTER(pmf_params), P, P, P, d, d, d, i32, P, P],
             "pmf_import_mhd_raw": [P, ctypes.c_char_p],
+            "pmf_advance": [P, i64, d, d, d, pd, pd],
             "pmf_export_mhd_raw": [P, ctypes.c_char_p],
             "pmf_max_wavespeed_of": [P, P, pd],
             "pmf_upload": [P, i32, P, i64],

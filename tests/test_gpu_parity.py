@@ -769,3 +769,44 @@ def test_uniform_state_is_a_fixed_point_bit_for_bit(dtype, recon, riemann):
     dt, sm = M.compute_dt(d, 0.3)
     M.step_drag(d, float(dt), float(sm), 0.99, 0.3, integrator="cube")
     assert np.array_equal(d.conserved(), r)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("fac,bcs", [(1.0, ("periodic",) * 3), (1.4, ("outflow", "reflecting", "periodic"))])
+def test_advance_with_device_dt_equals_host_driven_steps(dtype, fac, bcs):
+    """pmf_advance computes each dt, ch and psi decay on the device from the speed the
+    previous step folded (evolve!'s rule, driver.jl:48-71); the same steps driven from the
+    host (compute_dt, then step with ch = glm_ch_fac * smax and dt scaled by smax/max(smax,
+    ch), as evolve! does) give the same state: bit for bit in float32, to rounding of the
+    psi factor's exp in float64. The elapsed time is the sum of the host's dts."""
+    dims = (48, 40, 32)
+    U = smooth_state(dims, seed=31)
+    kw = dict(dtype=dtype, recon="plm_minmod", bcs=bcs)
+    a = M.allocate_state(dims, 1.0 / dims[0], **kw)
+    b = M.allocate_state(dims, 1.0 / dims[0], **kw)
+    a.set_conserved(U)
+    b.set_conserved(U)
+    t_host = 0.0
+    for _ in range(6):
+        dt, sm = M.compute_dt(a, 0.3)
+        if dtype == np.float32:
+            smf = np.float32(sm)
+            chf = np.float32(fac) * smf
+            ch = float(chf)
+            dtf = float(dt) * float(smf) / float(max(smf, chf))
+        else:
+            ch = fac * float(sm)
+            dtf = float(dt) * float(sm) / max(float(sm), ch)
+        M.step(a, dtf, ch=ch, integrator="cube")
+        t_host += dtf
+    t, last = M.advance(b, 6, cfl=0.3, glm_ch_fac=fac)
+    assert t == t_host and last == dtf
+    Ua, Ub = a.conserved(), b.conserved()
+    if dtype == np.float32:
+        assert np.array_equal(Ua, Ub)
+    else:
+        scale = np.abs(Ua).max(axis=1, keepdims=True)
+        assert np.max(np.abs(Ua - Ub) / scale) < 1e-14
+    assert M.compute_dt(b, 0.3)[1] == M.compute_dt(a, 0.3)[1]
+    with pytest.raises(M.MHDError, match="constrained-transport states step with pmf_step"):
+        M.advance(M.allocate_state((16,) * 3, 1 / 16, dtype=dtype, mhd_solver="ct"), 1)
