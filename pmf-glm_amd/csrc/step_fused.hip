@@ -325,6 +325,15 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         if (src.check == CHECK_MASKED) mkv = src.corr[4 * src.stride + pc];
       }
     }
+    // CT: the face fields this thread's solves take as the normal field (the z face m-1/2
+    // of an owned column, the x and y faces of plane m), also in flight during the barrier
+    T bfz = T(0), bfx = T(0), bfy = T(0);
+    if constexpr (CT) {
+      const int64_t n = g.nc;
+      if (owned && !first) bfz = ctp[2 * n + (int64_t)wrapi(m, g.nz) * g.plane + col];
+      if (ring_work && xsolver) bfx = ctp[(int64_t)m * g.plane + col];
+      if (ring_work && ysolver) bfy = ctp[n + (int64_t)m * g.plane + col];
+    }
     if constexpr (SURR) {
       if (t == 0) Qp->n = 0u;  // the previous plane's queue was last read before its final barrier
     }
@@ -470,9 +479,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
               // in both states; its mass and transverse-B fluxes go to the workspace
               // (integrator_ref.jl:83-132) when plane m belongs to this chunk
               const int64_t idx = (int64_t)wrapi(m, g.nz) * g.plane + col, n = g.nc;
-              const T bf = ctp[2 * n + idx];
-              Lzp[7] = bf;
-              Rz[7] = bf;
+              Lzp[7] = bfz;
+              Rz[7] = bfz;
               face_flux<2, true, T, RS>(Lzp, Rz, Fz, P);
               if (stores && !last) {
                 ctp[5 * n + idx] = Fz[0];
@@ -579,9 +587,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
         if constexpr (CT) {
           const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
-          const T bf = ctp[idx];
-          L[5] = bf;
-          Rx[5] = bf;
+          L[5] = bfx;
+          Rx[5] = bfx;
           face_flux<0, true, T, RS>(L, Rx, Fx, P);
           if (stores) {
             ctp[3 * n + idx] = Fx[0];
@@ -602,9 +609,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
         if constexpr (CT) {
           const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
-          const T bf = ctp[n + idx];
-          L[6] = bf;
-          Ry[6] = bf;
+          L[6] = bfy;
+          Ry[6] = bfy;
           face_flux<1, true, T, RS>(L, Ry, Fy, P);
           if (stores) {
             ctp[4 * n + idx] = Fy[0];
