@@ -223,77 +223,49 @@ __device__ __forceinline__ T gs05(T Ea0, T Eam, T Eb0, T Ebm, T mA0, T mAm, T mB
   return T(0.25) * ((Ea0 + Eam + Eb0 + Ebm) + (A3 - A1) + (B3 - B1));
 }
 
-// The CT passes walk the grid by rows: a block takes rows r = (k, j) in turn and its
-// threads the cells i of a row, so a cell's coordinates and its periodic neighbours come
-// from 32-bit compares instead of 64-bit divisions.
-struct CtRow {
-  int64_t base;          // linear index of cell (0, j, k)
-  int64_t djm, djp;      // offsets of rows j-1, j+1 (same k), periodic
-  int64_t dkm, dkp;      // offsets of planes k-1, k+1 (same j), periodic
-};
-__device__ __forceinline__ CtRow ct_row(const Grid& G, int r) {
-  const int j = r % G.ny, k = r / G.ny;
-  const int64_t nx = G.nx, plane = (int64_t)G.nx * G.ny;
-  CtRow w;
-  w.base = (int64_t)r * nx;
-  w.djm = j == 0 ? (G.ny - 1) * nx : -nx;
-  w.djp = j == G.ny - 1 ? -(G.ny - 1) * nx : nx;
-  w.dkm = k == 0 ? (G.nz - 1) * plane : -plane;
-  w.dkp = k == G.nz - 1 ? -(G.nz - 1) * plane : plane;
-  return w;
-}
-__device__ __forceinline__ int wrap_m(int i, int n) { return i == 0 ? n - 1 : i - 1; }
-__device__ __forceinline__ int wrap_p(int i, int n) { return i == n - 1 ? 0 : i + 1; }
-
 // ct.jl:71-93: edge EMFs from the lower-face fluxes and the cell EMFs.
 template <class T> __global__ __launch_bounds__(kBlk) void k_ct_emf(T* __restrict__ ctp, Grid G) {
   const int64_t n = G.nc;
+  int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+  if (c >= n) return;
+  Pc p{G.nx, G.ny, G.nz};
+  int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
   const T *mfx = ctp + 3 * n, *mfy = ctp + 4 * n, *mfz = ctp + 5 * n;
   const T *fxby = ctp + 6 * n, *fxbz = ctp + 7 * n, *fybx = ctp + 8 * n, *fybz = ctp + 9 * n;
   const T *fzbx = ctp + 10 * n, *fzby = ctp + 11 * n;
   const T *ecx = ctp + 12 * n, *ecy = ctp + 13 * n, *ecz = ctp + 14 * n;
-  const int rows = G.ny * G.nz;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const CtRow w = ct_row(G, r);
-    for (int i = threadIdx.x; i < G.nx; i += kBlk) {
-      const int64_t c = w.base + i;
-      const int64_t cim = w.base + wrap_m(i, G.nx), cjm = c + w.djm, ckm = c + w.dkm;
-      const int64_t cimjm = cim + w.djm, cjmkm = cjm + w.dkm, ckmim = cim + w.dkm;
-      ctp[17 * n + c] = gs05(-fxby[c], -fxby[cjm], fybx[c], fybx[cim], mfx[c], mfx[cjm], mfy[c], mfy[cim], ecz[c],
-                             ecz[cim], ecz[cjm], ecz[cimjm]);
-      ctp[15 * n + c] = gs05(-fybz[c], -fybz[ckm], fzby[c], fzby[cjm], mfy[c], mfy[ckm], mfz[c], mfz[cjm], ecx[c],
-                             ecx[cjm], ecx[ckm], ecx[cjmkm]);
-      ctp[16 * n + c] = gs05(-fzbx[c], -fzbx[cim], fxbz[c], fxbz[ckm], mfz[c], mfz[cim], mfx[c], mfx[ckm], ecy[c],
-                             ecy[ckm], ecy[cim], ecy[ckmim]);
-    }
-  }
+  int64_t cim = p.at(i - 1, j, k), cjm = p.at(i, j - 1, k), ckm = p.at(i, j, k - 1);
+  int64_t cimjm = p.at(i - 1, j - 1, k), cjmkm = p.at(i, j - 1, k - 1), ckmim = p.at(i - 1, j, k - 1);
+  ctp[17 * n + c] = gs05(-fxby[c], -fxby[cjm], fybx[c], fybx[cim], mfx[c], mfx[cjm], mfy[c], mfy[cim], ecz[c],
+                         ecz[cim], ecz[cjm], ecz[cimjm]);
+  ctp[15 * n + c] = gs05(-fybz[c], -fybz[ckm], fzby[c], fzby[cjm], mfy[c], mfy[ckm], mfz[c], mfz[cjm], ecx[c],
+                         ecx[cjm], ecx[ckm], ecx[cjmkm]);
+  ctp[16 * n + c] = gs05(-fzbx[c], -fzbx[cim], fxbz[c], fxbz[ckm], mfz[c], mfz[cim], mfx[c], mfx[ckm], ecy[c],
+                         ecy[ckm], ecy[cim], ecy[ckmim]);
 }
 
 // ct.jl:96-129: face update by the discrete curl (optionally Kahan-compensated).
 template <class T, bool COMP> __global__ __launch_bounds__(kBlk) void k_ct_faces(T* __restrict__ ctp, Grid G, T dtdx) {
   const int64_t n = G.nc;
+  int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+  if (c >= n) return;
+  Pc p{G.nx, G.ny, G.nz};
+  int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
   const T *ex = ctp + 15 * n, *ey = ctp + 16 * n, *ez = ctp + 17 * n;
-  const int rows = G.ny * G.nz;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const CtRow w = ct_row(G, r);
-    for (int i = threadIdx.x; i < G.nx; i += kBlk) {
-      const int64_t c = w.base + i;
-      const int64_t ip = w.base + wrap_p(i, G.nx), jp = c + w.djp, kp = c + w.dkp;
-      T inc[3] = {-(dtdx * ((ez[jp] - ez[c]) - (ey[kp] - ey[c]))), -(dtdx * ((ex[kp] - ex[c]) - (ez[ip] - ez[c]))),
-                  -(dtdx * ((ey[ip] - ey[c]) - (ex[jp] - ex[c])))};
+  int64_t ip = p.at(i + 1, j, k), jp = p.at(i, j + 1, k), kp = p.at(i, j, k + 1);
+  T inc[3] = {-(dtdx * ((ez[jp] - ez[c]) - (ey[kp] - ey[c]))), -(dtdx * ((ex[kp] - ex[c]) - (ez[ip] - ez[c]))),
+              -(dtdx * ((ey[ip] - ey[c]) - (ex[jp] - ex[c])))};
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        T b = ctp[d * n + c];
-        if constexpr (COMP) {
-          // glm_physics.jl:352-359
-          T incr = inc[d] + ctp[(18 + d) * n + c];
-          T upd = b + incr;
-          ctp[(18 + d) * n + c] = incr - (upd - b);
-          ctp[d * n + c] = upd;
-        } else {
-          ctp[d * n + c] = b + inc[d];
-        }
-      }
+  for (int d = 0; d < 3; ++d) {
+    T b = ctp[d * n + c];
+    if constexpr (COMP) {
+      // glm_physics.jl:352-359
+      T incr = inc[d] + ctp[(18 + d) * n + c];
+      T upd = b + incr;
+      ctp[(18 + d) * n + c] = incr - (upd - b);
+      ctp[d * n + c] = upd;
+    } else {
+      ctp[d * n + c] = b + inc[d];
     }
   }
 }
@@ -301,16 +273,13 @@ template <class T, bool COMP> __global__ __launch_bounds__(kBlk) void k_ct_faces
 // ct.jl:132-143: cell B = face average.
 template <class T> __global__ __launch_bounds__(kBlk) void k_ct_bcc(Fields<T> U, const T* __restrict__ ctp, Grid G) {
   const int64_t n = G.nc;
-  const int rows = G.ny * G.nz;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const CtRow w = ct_row(G, r);
-    for (int i = threadIdx.x; i < G.nx; i += kBlk) {
-      const int64_t c = w.base + i;
-      U.v[5][c] = T(0.5) * (ctp[c] + ctp[w.base + wrap_p(i, G.nx)]);
-      U.v[6][c] = T(0.5) * (ctp[n + c] + ctp[n + c + w.djp]);
-      U.v[7][c] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + c + w.dkp]);
-    }
-  }
+  int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+  if (c >= n) return;
+  Pc p{G.nx, G.ny, G.nz};
+  int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+  U.v[5][c] = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
+  U.v[6][c] = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
+  U.v[7][c] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
 }
 
 // ct.jl:132-143 and timestep.jl:8-12 in one pass: the cell field from the faces, then
@@ -319,41 +288,31 @@ template <class T>
 __global__ __launch_bounds__(kBlk) void k_ct_bcc_cfl(Fields<T> U, const T* __restrict__ ctp, Grid G, StepParams<T> P,
                                                      double* __restrict__ red) {
   const int64_t n = G.nc;
+  Pc p{G.nx, G.ny, G.nz};
   double m = 0.0;
-  const int rows = G.ny * G.nz;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const CtRow w = ct_row(G, r);
-    for (int i = threadIdx.x; i < G.nx; i += kBlk) {
-      const int64_t c = w.base + i;
-      T q[9];
+  for (int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x; c < n; c += (int64_t)gridDim.x * kBlk) {
+    int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+    T q[9];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) q[v] = U.v[v][c];
-      q[5] = T(0.5) * (ctp[c] + ctp[w.base + wrap_p(i, G.nx)]);
-      q[6] = T(0.5) * (ctp[n + c] + ctp[n + c + w.djp]);
-      q[7] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + c + w.dkp]);
-      q[8] = U.v[8][c];
-      U.v[5][c] = q[5];
-      U.v[6][c] = q[6];
-      U.v[7][c] = q[7];
-      m = nanmax(m, (double)cell_wavespeed(q, P));
-    }
+    for (int v = 0; v < 5; ++v) q[v] = U.v[v][c];
+    q[5] = T(0.5) * (ctp[c] + ctp[p.at(i + 1, j, k)]);
+    q[6] = T(0.5) * (ctp[n + c] + ctp[n + p.at(i, j + 1, k)]);
+    q[7] = T(0.5) * (ctp[2 * n + c] + ctp[2 * n + p.at(i, j, k + 1)]);
+    q[8] = U.v[8][c];
+    U.v[5][c] = q[5];
+    U.v[6][c] = q[6];
+    U.v[7][c] = q[7];
+    m = nanmax(m, (double)cell_wavespeed(q, P));
   }
   KSum none[1];
   block_fold<0, 1>(none, &m);
   store_partial<0, 1>(red, none, &m);
 }
 
-// Row-walking grid of the CT passes: one block per row up to a bound that keeps the
-// launch modest on large grids (each block then takes several rows).
-static inline unsigned ct_rows_grid(const Grid& G) {
-  const int64_t rows = (int64_t)G.ny * G.nz;
-  return (unsigned)(rows < 65536 ? (rows < 1 ? 1 : rows) : 65536);
-}
-
 template <class T>
 void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st,
                       const StepParams<T>* P, double* red, double* dsmax) {
-  dim3 blk(kBlk), grd(ct_rows_grid(G));
+  dim3 blk(kBlk), grd((unsigned)((G.nc + kBlk - 1) / kBlk));
   hipLaunchKernelGGL((k_ct_emf<T>), grd, blk, 0, st, ctp, G);
   if (compensate) hipLaunchKernelGGL((k_ct_faces<T, true>), grd, blk, 0, st, ctp, G, dtdx);
   else hipLaunchKernelGGL((k_ct_faces<T, false>), grd, blk, 0, st, ctp, G, dtdx);

@@ -225,6 +225,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   const bool xleft = has_col && ix < TX && iy >= 0 && iy < TY;     // owned + left ring column
   const bool yleft = has_col && iy < TY && ix >= 0 && ix < TX;     // owned + bottom ring row
   const int me = pidx(ix, iy);
+  // CT workspace slot q of cell idx (NARROW: the packed buffer's byte offsets fit 32 bits)
+  auto ctslot = [&](int q, int64_t idx) -> T& { return *cell_at<NARROW>(ctp + (int64_t)q * g.nc, idx); };
 
   // Column registers: Lzp = left state of the z face below the current plane, Fzp = flux
   // of the z face below the plane being updated, Dxy = that plane's x/y divergence (PPM:
@@ -260,13 +262,43 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   }
   T smax = T(0);
 
+  // CT workspace stores are issued after the loads that follow them in program order
+  // (a wave's vector-memory counter retires loads and stores in issue order, so a store
+  // ahead of a load makes the wait for the load also wait for the store's write): the
+  // cell EMF and z-face fluxes of plane m go out after the update's loads of plane m-1,
+  // and the x/y-face fluxes of plane m after the next plane's loads.
+  T ctc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};  // plane m: ecx ecy ecz, mfz fzbx fzby
+  bool ct_emf = false, ct_zf = false;
+  T ctxy[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};  // the previous plane: mfx fxby fxbz, mfy fybx fybz
+  auto ct_flush_cell = [&](int m) {
+    if constexpr (CT) {
+      const int64_t idx = (int64_t)wrapi(m, g.nz) * g.plane + col;
+      if (ct_emf) {
+        ctslot(12, idx) = ctc[0];
+        ctslot(13, idx) = ctc[1];
+        ctslot(14, idx) = ctc[2];
+      }
+      if (ct_zf) {
+        ctslot(5, idx) = ctc[3];
+        ctslot(10, idx) = ctc[4];
+        ctslot(11, idx) = ctc[5];
+      }
+      ct_emf = ct_zf = false;
+    }
+  };
+
   // (4) the update of plane m-1 once the z face m-1/2 is known (integrator_ref.jl:140-158);
   // shifts the z-face carry. Owned columns only.
   auto zface_update = [&](const T* Fz, const T* Lzn, int m, bool first) {
+    bool flushed = false;
     if (m - 1 >= z0 && stores) {
       T o[9];
       const int64_t idx = (int64_t)(m - 1) * g.plane + col;
       load_cons<NARROW>(U, idx, o);
+      if constexpr (CT) {
+        ct_flush_cell(m);
+        flushed = true;
+      }
       if constexpr (SRC) {
         // tracked increments, drag propagator, admissibility mark (integrator_cube.jl:423-441)
         T du[9], U0[9];
@@ -290,6 +322,9 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       // CT: the cell field is replaced by the face average after the step, so the CFL
       // speed is taken from the final state instead
       if constexpr (!CT) smax = nanmax(smax, cell_wavespeed(o, P));
+    }
+    if constexpr (CT) {
+      if (!flushed) ct_flush_cell(m);
     }
     if (!first) {
 #pragma unroll
@@ -329,10 +364,19 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     // of an owned column, the x and y faces of plane m), also in flight during the barrier
     T bfz = T(0), bfx = T(0), bfy = T(0);
     if constexpr (CT) {
-      const int64_t n = g.nc;
-      if (owned && !first) bfz = ctp[2 * n + (int64_t)wrapi(m, g.nz) * g.plane + col];
-      if (ring_work && xsolver) bfx = ctp[(int64_t)m * g.plane + col];
-      if (ring_work && ysolver) bfy = ctp[n + (int64_t)m * g.plane + col];
+      if (owned && !first) bfz = ctslot(2, (int64_t)wrapi(m, g.nz) * g.plane + col);
+      if (ring_work && xsolver) bfx = ctslot(0, (int64_t)m * g.plane + col);
+      if (ring_work && ysolver) bfy = ctslot(1, (int64_t)m * g.plane + col);
+      // the x/y-face fluxes of plane m-1 of the chunk's own cells, now behind the loads
+      if (stores && m - 1 >= z0 && m - 1 < z0 + zc) {
+        const int64_t idx = (int64_t)(m - 1) * g.plane + col;
+        ctslot(3, idx) = ctxy[0];
+        ctslot(6, idx) = ctxy[1];
+        ctslot(7, idx) = ctxy[2];
+        ctslot(4, idx) = ctxy[3];
+        ctslot(8, idx) = ctxy[4];
+        ctslot(9, idx) = ctxy[5];
+      }
     }
     if constexpr (SURR) {
       if (t == 0) Qp->n = 0u;  // the previous plane's queue was last read before its final barrier
@@ -455,7 +499,6 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         // radiation correction of a zero-offset face state) and the start-of-step field
         // (integrator_ref.jl:133-138), for the chunk's own cells
         if (stores && !first && !last) {
-          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
           T qc[9];
 #pragma unroll
           for (int v = 0; v < 9; ++v) qc[v] = mh[v];
@@ -463,9 +506,10 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
             const T zero[9] = {};
             src_face_pre(qc, m0, zero, src, cv);
           }
-          ctp[12 * n + idx] = qc[3] * m0[6] - qc[2] * m0[7];
-          ctp[13 * n + idx] = qc[1] * m0[7] - qc[3] * m0[5];
-          ctp[14 * n + idx] = qc[2] * m0[5] - qc[1] * m0[6];
+          ctc[0] = qc[3] * m0[6] - qc[2] * m0[7];
+          ctc[1] = qc[1] * m0[7] - qc[3] * m0[5];
+          ctc[2] = qc[2] * m0[5] - qc[1] * m0[6];
+          ct_emf = true;
         }
       }
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
@@ -478,14 +522,14 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
               // riemann_ct on the z face m-1/2 = the lower face of plane m: the face field
               // in both states; its mass and transverse-B fluxes go to the workspace
               // (integrator_ref.jl:83-132) when plane m belongs to this chunk
-              const int64_t idx = (int64_t)wrapi(m, g.nz) * g.plane + col, n = g.nc;
               Lzp[7] = bfz;
               Rz[7] = bfz;
               face_flux<2, true, T, RS>(Lzp, Rz, Fz, P);
               if (stores && !last) {
-                ctp[5 * n + idx] = Fz[0];
-                ctp[10 * n + idx] = Fz[5];
-                ctp[11 * n + idx] = Fz[6];
+                ctc[3] = Fz[0];
+                ctc[4] = Fz[5];
+                ctc[5] = Fz[6];
+                ct_zf = true;
               }
             } else if (SRC && src.check == CHECK_MASKED && (mkz || mkzp)) {
               face_flux<2, false, T, RS_LLF>(Lzp, Rz, Fz, P);
@@ -586,15 +630,12 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
         if constexpr (CT) {
-          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
           L[5] = bfx;
           Rx[5] = bfx;
           face_flux<0, true, T, RS>(L, Rx, Fx, P);
-          if (stores) {
-            ctp[3 * n + idx] = Fx[0];
-            ctp[6 * n + idx] = Fx[6];
-            ctp[7 * n + idx] = Fx[7];
-          }
+          ctxy[0] = Fx[0];
+          ctxy[1] = Fx[6];
+          ctxy[2] = Fx[7];
         } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)])) {
           face_flux<0, false, T, RS_LLF>(L, Rx, Fx, P);
         } else {
@@ -608,15 +649,12 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 #pragma unroll
         for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
         if constexpr (CT) {
-          const int64_t idx = (int64_t)m * g.plane + col, n = g.nc;
           L[6] = bfy;
           Ry[6] = bfy;
           face_flux<1, true, T, RS>(L, Ry, Fy, P);
-          if (stores) {
-            ctp[4 * n + idx] = Fy[0];
-            ctp[8 * n + idx] = Fy[5];
-            ctp[9 * n + idx] = Fy[7];
-          }
+          ctxy[3] = Fy[0];
+          ctxy[4] = Fy[5];
+          ctxy[5] = Fy[7];
         } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)])) {
           face_flux<1, false, T, RS_LLF>(L, Ry, Fy, P);
         } else {
@@ -1132,7 +1170,7 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 // the face fields, integrator_ref.jl:83-139) with each face solved once; writes the cell
 // update and workspace slots 3-14 (face mass and transverse-B fluxes of every cell's
 // lower faces, cell EMFs) that ct_update! reads.
-template <class T, int RECON, int RS, bool SRC = false>
+template <class T, int RECON, int RS, bool SRC = false, bool NARROW = false>
 __device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, const Fields<T>& O, T* __restrict__ ctp,
                                                    const FusedGeom& g, const StepParams<T>& P,
                                                    const SrcParams<T>& src = SrcParams<T>{}) {
@@ -1141,8 +1179,8 @@ __device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, con
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
   unsigned none = 0;
-  (void)fused_item<T, RECON, RS, false, SRC, false, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, src,
-                                                          nullptr, ctp);
+  (void)fused_item<T, RECON, RS, false, SRC, NARROW, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, src,
+                                                           nullptr, ctp);
 }
 
 template <class T, int RECON, int RS>
@@ -1159,10 +1197,12 @@ constexpr bool ct_three_waves() {
   return !(RECON == RECON_PLM_MINMOD && RS == RS_LLF);
 }
 
-template <int RECON, int RS>
+// NARROW: every field and every slot of the packed buffer is addressed as its scalar base
+// plus a 32-bit byte offset (fields below 4 GiB).
+template <int RECON, int RS, bool NARROW>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused_ct3(
     Fields<const float> U, Fields<float> O, float* __restrict__ ctp, FusedGeom g, StepParams<float> P) {
-  step_fused_ct_body<float, RECON, RS>(U, O, ctp, g, P);
+  step_fused_ct_body<float, RECON, RS, false, NARROW>(U, O, ctp, g, P);
 }
 
 // The CT step with the drag / radiation sources (step_ref_drag! / step_ref_radiation! on
@@ -1180,13 +1220,17 @@ bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G,
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve) || !all_periodic(G)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
+  const bool narrow = sizeof(T) == 4 && narrow_fields<T>(G.nc);
 #define PMF_CT(R, S)                                                                   \
   do {                                                                                 \
     if (src)                                                                           \
       hipLaunchKernelGGL((k_step_fused_ct_src<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P, *src); \
-    else if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>())                      \
-      hipLaunchKernelGGL((k_step_fused_ct3<R, S>), grd, blk, 0, st, U, O, ctp, g, P);  \
-    else                                                                               \
+    else if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>()) {                    \
+      if (narrow)                                                                      \
+        hipLaunchKernelGGL((k_step_fused_ct3<R, S, true>), grd, blk, 0, st, U, O, ctp, g, P); \
+      else                                                                             \
+        hipLaunchKernelGGL((k_step_fused_ct3<R, S, false>), grd, blk, 0, st, U, O, ctp, g, P); \
+    } else                                                                             \
       hipLaunchKernelGGL((k_step_fused_ct<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P); \
   } while (0)
 #define PMF_CT_RS(R)                                \

@@ -520,6 +520,28 @@ def test_fused_float32_wide_offsets_equal_narrow_offsets(recon, monkeypatch):
     assert out[0][1] == out[1][1]
 
 
+@pytest.mark.parametrize("recon,riemann", [("plm_minmod", "hlld"), ("ppm", "hll")])
+def test_ct_fused_float32_wide_offsets_equal_narrow_offsets(recon, riemann, monkeypatch):
+    """The fused CT kernel's 32-bit-offset instance (fields and workspace slots as scalar
+    bases plus 32-bit byte offsets) against its 64-bit-offset instance: cells and all 21
+    CT slots bit for bit."""
+    from test_oracle import ct_test_hydro, ct_test_potential
+    n = 24
+    out = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("PMFMHD_WIDE_OFFSETS", wide)
+        s = M.allocate_state((n,) * 3, 1.0 / n, dtype=np.float32, recon=recon, riemann=riemann, mhd_solver="ct")
+        s.set_conserved(ct_test_hydro(n, G))
+        M.ct_faces_from_potential(s, *ct_test_potential(n))
+        for _ in range(3):
+            dt, _ = M.compute_dt(s, 0.3)
+            M.step(s, float(dt))
+        out.append((s.conserved(), [s.ct_slot(q) for q in range(21)]))
+    assert np.array_equal(out[0][0], out[1][0])
+    for q in range(21):
+        assert np.array_equal(out[0][1][q], out[1][1][q]), q
+
+
 def test_fused_rejects_a_one_cell_reflecting_axis():
     b = M.allocate_state((16, 16, 1), 1 / 16, bcs=("periodic", "periodic", "reflecting"))
     with pytest.raises(M.MHDError, match="reflecting axes two"):
