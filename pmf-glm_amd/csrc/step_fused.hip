@@ -182,6 +182,10 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   const bool owned = t < NOWN;
   // ragged last tile column / row: computed (their cells feed no stored cell), not stored
   const bool stores = owned && x0 + ix < g.nx && y0 + iy < g.ny;
+  // live extent of the tile: a ragged last tile (fewer than TX columns or TY rows left in
+  // the grid) solves only the faces of its stored cells and predicts only their states, so
+  // the waves over its dead rows skip their work
+  const int vx = min(TX, g.nx - x0), vy = min(TY, g.ny - y0);
   // the column's periodic image: its source cells on periodic grids, and the index of the
   // radiation correction buffer on every grid (integrator_ref.jl:50-53)
   const int64_t pcol = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
@@ -220,8 +224,11 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       load_cons<NARROW>(U, (int64_t)wrapi(z, g.nz) * g.plane + c2, c);
     }
   };
-  const bool xsolver = has_col && iy >= 0 && iy < TY && ix >= 0;  // owned + right ring column
-  const bool ysolver = has_col && ix >= 0 && ix < TX && iy >= 0;  // owned + top ring row
+  const bool xsolver = has_col && iy >= 0 && iy < vy && ix >= 0 && ix <= vx;  // owned + right ring column
+  const bool ysolver = has_col && ix >= 0 && ix < vx && iy >= 0 && iy <= vy;  // owned + top ring row
+  // columns whose predicted states feed a live face (every column of a whole tile)
+  const bool pred_live =
+      has_col && ((ix >= -1 && ix <= vx && iy >= 0 && iy < vy) || (ix >= 0 && ix < vx && iy >= -1 && iy <= vy));
   const bool xleft = has_col && ix < TX && iy >= 0 && iy < TY;     // owned + left ring column
   const bool yleft = has_col && iy < TY && ix >= 0 && ix < TX;     // owned + bottom ring row
   const int me = pidx(ix, iy);
@@ -364,7 +371,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     // of an owned column, the x and y faces of plane m), also in flight during the barrier
     T bfz = T(0), bfx = T(0), bfy = T(0);
     if constexpr (CT) {
-      if (owned && !first) bfz = ctslot(2, (int64_t)wrapi(m, g.nz) * g.plane + col);
+      if (stores && !first) bfz = ctslot(2, (int64_t)wrapi(m, g.nz) * g.plane + col);
       if (ring_work && xsolver) bfx = ctslot(0, (int64_t)m * g.plane + col);
       if (ring_work && ysolver) bfy = ctslot(1, (int64_t)m * g.plane + col);
       // the x/y-face fluxes of plane m-1 of the chunk's own cells, now behind the loads
@@ -396,7 +403,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     }
     // (2) Hancock prediction of plane m (glm_physics.jl:225-231, integrator_ref.jl:17-26)
     T Rx[9], Ry[9], Rz[9], Lzn[9];
-    if (owned || (has_col && ring_work)) {
+    if (stores || (pred_live && ring_work)) {
       T m0[9], dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
@@ -515,7 +522,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       // (3a) z face m-1/2 (both states in registers), then (4) the update of plane m-1,
       // whose inputs all exist now (integrator_ref.jl:140-158)
       if constexpr (!SURR) {
-        if (owned) {
+        if (stores) {
           T Fz[9];
           if (!first) {
             if constexpr (CT) {
@@ -557,7 +564,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       // (c) each smooth face adds its learned dissipation to its central flux. Face
       // states were read in (a), so the fluxes may overwrite them after (b)'s barriers.
       SurrQueue<FQCAP>& Q = *Qp;
-      const bool zf = owned && !first, xf = xsolver && xy, yf = ysolver && xy;
+      const bool zf = stores && !first, xf = xsolver && xy, yf = ysolver && xy;
       float fz[9], sz[9], fx[9], sx[9], fy[9], sy[9];
       int qz = -1, qx = -1, qy = -1;
       if (zf) {
@@ -584,7 +591,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       for (int b = (t >> 6) * 32; b < nq; b += FTHREADS / 2) surr_batch(b, nq, Q, sW, sB);
       __syncthreads();
       float y[SURR_OUT];
-      if (owned) {
+      if (stores) {
         T Fz[9];
         if (zf) {
           surr_result(Q, qz, y);
@@ -612,7 +619,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
           for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy - 1)] = Fy[v];
         }
         __syncthreads();
-        if (owned) {
+        if (stores) {
 #pragma unroll
           for (int v = 0; v < 9; ++v)
             dxy(v) = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
@@ -665,7 +672,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       }
       }
       __syncthreads();
-      if (owned) {
+      if (stores) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
           dxy(v) = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
