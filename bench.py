@@ -111,7 +111,7 @@ def replay(M, U, n, seq, surrogate=None, thr=None):
 
 def error_report(M, P, n, steps):
     """Float32 plain step against float64 on the same (dt, ch) sequence (oracle-dt replay)."""
-    U = P.config_state("c2_256", seed=1, n=n)
+    U = P.config_state("turb_256", seed=1, n=n)
     ref, seq = fp64_sequence(M, U, n, steps)
     got, _, _ = replay(M, U, n, seq)
     return {"workload": f"plain GLM-HLLD {n}^3, seed 1, {steps} CFL-0.2 steps", "steps": steps,
@@ -127,7 +127,7 @@ def oracle_error_report(M, P, n, steps):
     restatement of the reference's :ref kernel, OpenMP over the host cores."""
     import numpy as np
     import oracle as O
-    U = P.config_state("c2_256", seed=1, n=n)
+    U = P.config_state("turb_256", seed=1, n=n)
     s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
     s.set_conserved(U)
     seq = []
@@ -158,7 +158,7 @@ def fp64_oracle_check(M, P, n, steps):
     (dt, ch) sequence replayed by the oracle (OpenMP over the host cores)."""
     import numpy as np
     import oracle as O
-    U = P.config_state("c2_256", seed=1, n=n)
+    U = P.config_state("turb_256", seed=1, n=n)
     gpu, seq = fp64_sequence(M, U, n, steps)
     s = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
     s.set_conserved(U)
@@ -177,7 +177,7 @@ def surrogate_leg(M, P, n, steps, weights, thr):
     (rms |B| 0.3, |m| <= 8, v = 0): throughput of both, and each one's whole-run error
     against float64 on the same (dt, ch) sequence."""
     import numpy as np
-    U = P.config_state("c4_256", seed=1, n=n)
+    U = P.config_state("bstatic_256", seed=1, n=n)
     ref, seq = fp64_sequence(M, U, n, steps)
     plain, t_plain, _ = replay(M, U, n, seq)
     sur, t_sur, counts = replay(M, U, n, seq, surrogate=weights, thr=thr)
@@ -209,7 +209,7 @@ def surrogate_profile(M, P, n, steps, warmup, weights, thr):
     import numpy as np
     from pmfmhd.surrogate import Surrogate, step_surrogate
     s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
-    s.set_conserved(P.config_state("c4_256", seed=1, n=n))
+    s.set_conserved(P.config_state("bstatic_256", seed=1, n=n))
     net = Surrogate(s, seed=0)
     net.set_weights(np.load(weights))
     for _ in range(warmup):
@@ -278,12 +278,12 @@ def ledger(M, s):
     return {"totals": tot, "divb_l2": l2, "divb_linf": linf}
 
 
-def c3_leg(M, P, steps):
+def mach15_leg(M, P, steps):
     """512^3 plain step (Mach-1.5 turbulence, lognormal density): throughput and the
     float64 compensated ledger and div B norm before and after."""
     import numpy as np
     n = 512
-    U = P.config_state("c3_512", seed=1)
+    U = P.config_state("mach15_512", seed=1)
     s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
     s.set_conserved(U)
     del U
@@ -308,7 +308,7 @@ def ct_leg(M, P, n, steps):
     inputs as the plain run, after the divergence-free face projection; the per-cell
     kernel (the reference's structure) is timed on the same state for comparison."""
     import numpy as np
-    U = P.config_state("c2_256", seed=1, n=n)
+    U = P.config_state("turb_256", seed=1, n=n)
     s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod", mhd_solver="ct")
     s.set_conserved(U)
     proj = M.ct_faces_from_cells(s)
@@ -461,7 +461,7 @@ def main():
                                                                   "surrogate_weights.npy"))
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in z-chunk depth")
-    ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
+    ap.add_argument("--mach15-steps", type=int, default=100, help="steps of the 512^3 leg (0 = skip)")
     ap.add_argument("--fp64-oracle-steps", type=int, default=5,
                     help="steps of the float64 GPU-vs-CPU-oracle check at full size (0 = skip; ~3 s per step)")
     ap.add_argument("--oracle-error-n", type=int, default=128,
@@ -497,7 +497,7 @@ def main():
         return
     log(f"bench: building the {n}^3 initial state")
     t0 = time.perf_counter()
-    U = P.config_state("c2_256", seed=1, n=n)
+    U = P.config_state("turb_256", seed=1, n=n)
     log(f"bench: initial state ready ({time.perf_counter() - t0:.1f} s)")
     s = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod", riemann="hlld")
     s.set_conserved(U)
@@ -594,7 +594,7 @@ def main():
     if not args.no_ensemble:
         log("bench: ensemble leg")
         ens = ensemble_leg(P)
-    c3 = ct = orc = oerr = None
+    m15 = ct = orc = oerr = None
     if args.oracle_error_n > 0 and args.oracle_error_steps > 0 and integ == "cube":
         log(f"bench: float32 GPU vs float64 CPU oracle error report ({args.oracle_error_n}^3, "
             f"{args.oracle_error_steps} steps)")
@@ -602,9 +602,9 @@ def main():
     if args.fp64_oracle_steps > 0 and integ == "cube":
         log(f"bench: float64 GPU vs CPU oracle ({args.fp64_oracle_steps} steps)")
         orc = fp64_oracle_check(M, P, n, args.fp64_oracle_steps)
-    if args.c3_steps > 0:
-        log(f"bench: 512^3 leg ({args.c3_steps} steps)")
-        c3 = c3_leg(M, P, args.c3_steps)
+    if args.mach15_steps > 0:
+        log(f"bench: 512^3 leg ({args.mach15_steps} steps)")
+        m15 = mach15_leg(M, P, args.mach15_steps)
     if args.ct_steps > 0:
         log(f"bench: CT leg ({args.ct_steps} steps)")
         ct = ct_leg(M, P, n, args.ct_steps)
@@ -648,7 +648,7 @@ def main():
         "error_report": dict(oerr or {}, float64_gpu_reference_at_benchmark_size=errors),
         "surrogate": sur,
         "ensemble": ens,
-        "c3_512": c3,
+        "mach15_512": m15,
         "ct": ct,
         "fp64_oracle_check": orc,
         "coupled_steps": coupled,

@@ -59,7 +59,8 @@ __device__ __forceinline__ double dabs(double a) { return __builtin_fabs(a); }
 // function (column cells, halo cells, chunk start), so its arithmetic is pinned: explicit
 // FMAs and no other contraction. A cell then gets the same primitives whichever copy
 // converts it, and the step's result does not depend on where the tile seams fall.
-template <class T> __device__ __forceinline__ void cons2prim(const T* c, T* q, const StepParams<T>& P) {
+// Returns 1/rho (rho = q[0]) for callers that need it again.
+template <class T> __device__ __forceinline__ T cons2prim(const T* c, T* q, const StepParams<T>& P) {
 #pragma clang fp contract(off)
   T rho = dmax(c[0], P.smallr);
   T ir = rcp(rho);
@@ -74,6 +75,7 @@ template <class T> __device__ __forceinline__ void cons2prim(const T* c, T* q, c
   q[6] = c[6];
   q[7] = c[7];
   q[8] = c[8];
+  return ir;
 }
 
 // prim2cons (glm_physics.jl:19-23).
@@ -375,8 +377,7 @@ __device__ __forceinline__ void face_flux(const T* Lq, const T* Rq, T* F, const 
 // Per-cell CFL speed max_d(|v_d| + c_fast,d) of a conserved state (timestep.jl:8-12).
 template <class T> __device__ __forceinline__ T cell_wavespeed(const T* c, const StepParams<T>& P) {
   T q[9];
-  cons2prim(c, q, P);
-  T ir = rcp(q[0]);
+  const T ir = cons2prim(c, q, P);  // 1/q[0]
   T b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
   T sx = dabs(q[1]) + cfast(ir, q[4], b2, q[5], P.gamma);
   T sy = dabs(q[2]) + cfast(ir, q[4], b2, q[6], P.gamma);

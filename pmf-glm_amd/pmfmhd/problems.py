@@ -254,12 +254,12 @@ def random_mhd(n, seed=1, kmax=None, brms=0.3, b_index=2.0, vrms_mach=0.0, v_ind
 
 # The benchmark configurations (unit box, gamma = 5/3, PLM minmod, HLLD, CFL 0.2).
 CONFIGS = {
-    "c1_32": dict(n=32, kmax=8, brms=0.3, b_index=2.0, vrms_mach=0.0, rho_mode="gauss", rho_sigma=0.01),
-    "c2_256": dict(n=256, kmax=32, brms=1.0, b_index=2.0, vrms_mach=0.5, v_index=-2.0, rho_mode="gauss",
+    "bstatic_32": dict(n=32, kmax=8, brms=0.3, b_index=2.0, vrms_mach=0.0, rho_mode="gauss", rho_sigma=0.01),
+    "turb_256": dict(n=256, kmax=32, brms=1.0, b_index=2.0, vrms_mach=0.5, v_index=-2.0, rho_mode="gauss",
                    rho_sigma=0.01),
-    "c3_512": dict(n=512, kmax=64, brms=1.0, b_index=2.0, vrms_mach=1.5, v_index=-2.0, rho_mode="lognormal",
+    "mach15_512": dict(n=512, kmax=64, brms=1.0, b_index=2.0, vrms_mach=1.5, v_index=-2.0, rho_mode="lognormal",
                    rho_sigma=0.3),
-    "c4_256": dict(n=256, kmax=8, brms=0.3, b_index=2.0, vrms_mach=0.0, rho_mode="gauss", rho_sigma=0.01),
+    "bstatic_256": dict(n=256, kmax=8, brms=0.3, b_index=2.0, vrms_mach=0.0, rho_mode="gauss", rho_sigma=0.01),
 }
 
 

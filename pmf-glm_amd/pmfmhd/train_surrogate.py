@@ -96,8 +96,8 @@ def main(argv=None):
     half = args.pairs // 2
     comp = {}
     _log("dumping training pairs")
-    comp["c1_32"] = dump_runs(net, "c1_32", seeds, half, steps=64)
-    comp["c2_256"] = dump_runs(net, "c2_256", seeds, args.pairs - net.count, steps=4)
+    comp["bstatic_32"] = dump_runs(net, "bstatic_32", seeds, half, steps=64)
+    comp["turb_256"] = dump_runs(net, "turb_256", seeds, args.pairs - net.count, steps=4)
     _log(f"dataset: {net.count} pairs {comp} ({time.time() - t0:.0f} s)")
     losses = []
     for phase in args.schedule.split(","):
@@ -112,9 +112,9 @@ def main(argv=None):
     val = Surrogate(host, seed=1)
     val.set_weights(w)
     val.reserve(1 << 21)
-    dump_runs(val, "c1_32", [17], 1 << 20, steps=16)
+    dump_runs(val, "bstatic_32", [17], 1 << 20, steps=16)
     n1 = val.count
-    dump_runs(val, "c2_256", [17], 1 << 20, steps=2)
+    dump_runs(val, "turb_256", [17], 1 << 20, steps=2)
     report = {
         "pairs": int(net.count),
         "composition": comp,
@@ -123,8 +123,8 @@ def main(argv=None):
         "schedule": args.schedule,
         "final_loss": float(np.mean(losses[-500:])),
         "loss_every_100": [float(np.mean(losses[i:i + 100])) for i in range(0, len(losses), 100)],
-        "heldout_c1_32": face_errors(val, 0, n1),
-        "heldout_c2_256": face_errors(val, n1, val.count - n1),
+        "heldout_bstatic_32": face_errors(val, 0, n1),
+        "heldout_turb_256": face_errors(val, n1, val.count - n1),
         "seconds": time.time() - t0,
     }
     np.save(os.path.join(args.out, "surrogate_weights.npy"), w)

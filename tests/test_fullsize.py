@@ -14,7 +14,7 @@ N = 256
 
 @pytest.fixture(scope="module")
 def state256():
-    return P.config_state("c2_256", seed=1).astype(np.float32)
+    return P.config_state("turb_256", seed=1).astype(np.float32)
 
 
 def _state(U, **kw):
@@ -121,14 +121,14 @@ def test_species_transport_at_full_size(state256):
 
 
 @pytest.mark.timeout(900)
-def test_config3_512_fused_equals_per_cell_and_keeps_the_ledger():
+def test_mach15_512_fused_equals_per_cell_and_keeps_the_ledger():
     """512^3, the bandwidth-roofline configuration (Mach-1.5 solenoidal velocity, lognormal
     rho sigma 0.3, |m| <= 64, rms|B| 1): one fused step equals the per-cell kernel's to
     2e-6 of each field's maximum; five fused steps keep every field finite, the density
     positive, and the float64 ledger's mass and energy totals to 1e-9."""
     n = 512
-    print("config3 512^3: building the initial state", flush=True)
-    U = P.config_state("c3_512", seed=1).astype(np.float32)
+    print("mach15 512^3: building the initial state", flush=True)
+    U = P.config_state("mach15_512", seed=1).astype(np.float32)
     a = M.allocate_state((n,) * 3, 1.0 / n, dtype=np.float32, recon="plm_minmod")
     b = M.allocate_state((n,) * 3, 1.0 / n, dtype=np.float32, recon="plm_minmod")
     a.set_conserved(U)
@@ -160,7 +160,7 @@ def test_surrogate_step_at_full_size_without_gated_faces_is_the_plain_step():
     closed (thr = 0) no face takes the network and the surrogate kernel's step equals the
     plain fused step to float32 rounding, with the same CFL speed."""
     from pmfmhd.surrogate import Surrogate, step_surrogate
-    U = P.config_state("c4_256", seed=1).astype(np.float32)
+    U = P.config_state("bstatic_256", seed=1).astype(np.float32)
     a, b = _state(U), _state(U)
     net = Surrogate(b, seed=1)
     dt, smax = M.compute_dt(a, 0.2)

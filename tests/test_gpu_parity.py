@@ -217,7 +217,7 @@ def test_ref_step_boundaries_match_oracle(bcs):
 def test_config1_one_step_float32_vs_float32_oracle():
     """Acceptance bar: float32 GPU vs the float32 CPU run <= 1e-5 after one step."""
     n = 32
-    U = P.config_state("c1_32", seed=1)
+    U = P.config_state("bstatic_32", seed=1)
     g, o = pair((n, n, n), U, dtype=np.float32, recon="plm_minmod")
     dt, smax = o.compute_dt(0.2)
     o.step(float(dt), ch=smax)
@@ -230,7 +230,7 @@ def test_config1_one_step_float32_vs_float32_oracle():
 def test_config1_50_steps_float32_vs_float64_oracle():
     """The 32^3 benchmark run (50 CFL steps, cfl 0.2): each run computes its own dt."""
     n = 32
-    U = P.config_state("c1_32", seed=1)
+    U = P.config_state("bstatic_32", seed=1)
     g = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
     g.set_conserved(U)
     o = O.OracleState((n, n, n), 1.0 / n, dtype=np.float64, recon="plm_minmod")
@@ -282,7 +282,7 @@ def test_diagnostics_match_oracle():
 
 def test_float32_totals_are_float64_accurate():
     n = 32
-    U = P.config_state("c1_32", seed=2)
+    U = P.config_state("bstatic_32", seed=2)
     g = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32)
     g.set_conserved(U)
     tg = M.conserved_totals(g)
@@ -415,7 +415,7 @@ def test_ct_fused_kernel_float32_matches_per_cell_kernel():
     s = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, recon="plm_minmod", riemann="hlld", mhd_solver="ct")
     p = M.allocate_state((N, N, N), 1.0 / N, dtype=np.float32, recon="plm_minmod", riemann="hlld", mhd_solver="ct")
     M.set_ct_kernel(p, True)
-    U = P.config_state("c2_256", seed=2, n=N).astype(np.float32)
+    U = P.config_state("turb_256", seed=2, n=N).astype(np.float32)
     for st in (s, p):
         st.set_conserved(U)
         M.ct_faces_from_cells(st)
@@ -474,7 +474,7 @@ def test_fused_float64_other_solvers(solver):
 
 def test_fused_float32_config1_gates():
     n = 32
-    U = P.config_state("c1_32", seed=1)
+    U = P.config_state("bstatic_32", seed=1)
     g, o = pair((n, n, n), U, dtype=np.float32, recon="plm_minmod")
     r = M.allocate_state((n, n, n), 1.0 / n, dtype=np.float32, recon="plm_minmod")
     r.set_conserved(U)
@@ -699,7 +699,7 @@ def test_fused_chunk_depth_and_autotune_leave_results_unchanged():
     autotuner picks one of its candidates without touching the state."""
     import pmfmhd as M
     from pmfmhd import problems as P
-    U = P.config_state("c1_32", seed=3, n=48)
+    U = P.config_state("bstatic_32", seed=3, n=48)
     n = 48
     outs = []
     for zc in (0, 4, 7, 48, 200):
@@ -730,7 +730,7 @@ def test_ct_chunk_depth_and_autotune_leave_results_unchanged():
     """The same for the fused CT step: the tuner writes only scratch fields and per-step
     workspace, so cells and faces are untouched, and every depth gives the same bits."""
     n = 48
-    U = P.config_state("c2_256", seed=4, n=n).astype(np.float32)
+    U = P.config_state("turb_256", seed=4, n=n).astype(np.float32)
 
     def ct_state(zc):
         s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod", mhd_solver="ct")

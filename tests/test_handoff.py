@@ -37,7 +37,7 @@ def test_export_import_round_trip_and_layout(tmp_path, dtype):
     import pmfmhd as M
     from pmfmhd import problems as P
     dims = (12, 8, 6)
-    U = P.config_state("c1_32", seed=5, n=None)[:, : np.prod(dims)].astype(dtype)
+    U = P.config_state("bstatic_32", seed=5, n=None)[:, : np.prod(dims)].astype(dtype)
     a = M.allocate_state(dims, 1 / 12, dtype=dtype)
     a.set_conserved(U)
     M.export_mhd_raw(a, tmp_path)

@@ -15,8 +15,8 @@ G = 5.0 / 3.0
 N = 32
 
 
-def config3_state(seed=2, n=N):
-    return P.config_state("c3_512", seed=seed, n=n)
+def mach15_state(seed=2, n=N):
+    return P.config_state("mach15_512", seed=seed, n=n)
 
 
 def branch_census(U, n, dt, ch, llf_dmin=1e-4, recon="plm_minmod"):
@@ -42,8 +42,8 @@ def _dt(U, n, llf_dmin):
     return o.compute_dt(0.2)
 
 
-def test_config3_faces_reach_every_hlld_branch_and_the_llf_switch():
-    U = config3_state()
+def test_mach15_faces_reach_every_hlld_branch_and_the_llf_switch():
+    U = mach15_state()
     rho = U[0]
     assert rho.min() > 0.2 and abs(rho.mean() - 1) < 0.05  # lognormal, unit mean
     dt, smax = _dt(U, N, 1e-4)
@@ -58,12 +58,12 @@ def test_config3_faces_reach_every_hlld_branch_and_the_llf_switch():
 @pytest.mark.gpu
 @pytest.mark.parametrize("integrator", ["cube", "ref"])
 @pytest.mark.parametrize("llf_dmin", [1e-4, 0.8])
-def test_config3_float64_ten_steps_match_oracle(integrator, llf_dmin):
+def test_mach15_float64_ten_steps_match_oracle(integrator, llf_dmin):
     """float64 HIP against the float64 oracle to round-off over 10 CFL steps on the
     shock-forming workload, with the default LLF switch and with one that takes every face
     next to a density below 0.8."""
     import pmfmhd as M
-    U = config3_state()
+    U = mach15_state()
     kw = dict(recon="plm_minmod", llf_dmin=llf_dmin)
     g = M.allocate_state((N,) * 3, 1.0 / N, dtype=np.float64, **kw)
     o = O.OracleState((N,) * 3, 1.0 / N, **kw)
@@ -83,9 +83,9 @@ def test_config3_float64_ten_steps_match_oracle(integrator, llf_dmin):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("integrator", ["cube", "ref"])
-def test_config3_float32_one_step_matches_float32_oracle(integrator):
+def test_mach15_float32_one_step_matches_float32_oracle(integrator):
     import pmfmhd as M
-    U = config3_state().astype(np.float32)
+    U = mach15_state().astype(np.float32)
     g = M.allocate_state((N,) * 3, 1.0 / N, dtype=np.float32, recon="plm_minmod")
     o = O.OracleState((N,) * 3, 1.0 / N, dtype=np.float32, recon="plm_minmod")
     g.set_conserved(U)

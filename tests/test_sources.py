@@ -425,7 +425,7 @@ def test_radiation_float32_against_float32_oracle(integ):
     import pmfmhd as M
     from pmfmhd import problems as P
     n = 32
-    U = P.config_state("c1_32", seed=4).astype(np.float32)
+    U = P.config_state("bstatic_32", seed=4).astype(np.float32)
     s, o = _pair(n, U, np.float32, "plm_minmod", "hlld")
     c = _corr(n, U, seed=5, dtype=np.float32)
     buf = M.correction_buffer(s, c)

@@ -88,7 +88,7 @@ def test_planar_invariance_after_a_step(axis, integrator):
     solenoidal), boosted by (0.1, 0.2, 0.3), stays invariant along that axis after a step:
     every cell of a line sees the same arithmetic, so exactly."""
     N = 32
-    U = P.config_state("c2_256", seed=5, n=N).reshape(9, N, N, N)
+    U = P.config_state("turb_256", seed=5, n=N).reshape(9, N, N, N)
     ax = 3 - axis  # array axes are (var, z, y, x); axis 0 = x
     U = np.broadcast_to(U.mean(axis=ax, keepdims=True), U.shape).copy()
     rho = U[0]

@@ -71,7 +71,7 @@ def test_dump_faces_match_oracle_riemann_and_training_beats_central(gpu_state):
     net.reserve(400000)
     total = 0
     for seed in (1, 2, 3):
-        s.set_conserved(P.config_state("c1_32", seed=seed))
+        s.set_conserved(P.config_state("bstatic_32", seed=seed))
         for it in range(6):
             dt, smax = M.compute_dt(s, 0.2)
             total = net.dump(s, float(dt), float(smax), keep=0.5, seed=100 * seed + it)
@@ -165,7 +165,7 @@ def test_surrogate_encoding_closed_forms_and_gate_edges():
 
 def test_surrogate_step_oracle_without_gated_faces_is_the_plain_step():
     from pmfmhd import problems as P
-    U = P.config_state("c1_32", seed=4, n=16).astype(np.float64)
+    U = P.config_state("bstatic_32", seed=4, n=16).astype(np.float64)
     theta = np.random.default_rng(0).normal(0, 0.3, R.nparams())
     out, gates, _ = SN.step_periodic(U, (16, 16, 16), 0.1, 1.5, 0.97, theta, 0.0)
     ref = FN.step_periodic(U, (16, 16, 16), 0.1, 1.5, 0.97, G, recon="plm_minmod")
@@ -190,7 +190,7 @@ def test_dump_inputs_and_targets_match_oracle_encoding(gpu_state):
     s = gpu_state
     net = Surrogate(s, seed=2)
     net.reserve(200000)
-    s.set_conserved(P.config_state("c1_32", seed=8))
+    s.set_conserved(P.config_state("bstatic_32", seed=8))
     dt, smax = M.compute_dt(s, 0.2)
     n = net.dump(s, float(dt), float(smax), keep=1.0, seed=1)
     assert n > 5000
@@ -236,7 +236,7 @@ def test_surrogate_step_without_gated_faces_equals_plain_fused_step():
     import pmfmhd as M
     from pmfmhd import problems as P
     from pmfmhd.surrogate import Surrogate, step_surrogate
-    U = P.config_state("c1_32", seed=6)
+    U = P.config_state("bstatic_32", seed=6)
     a = M.allocate_state((32, 32, 32), 1 / 32, dtype=np.float32, recon="plm_minmod")
     b = M.allocate_state((32, 32, 32), 1 / 32, dtype=np.float32, recon="plm_minmod")
     a.set_conserved(U)
@@ -259,7 +259,7 @@ def test_surrogate_step_matches_float64_oracle(thr):
     from pmfmhd import problems as P
     from pmfmhd.surrogate import Surrogate, step_surrogate
     n = 32
-    U = P.config_state("c1_32", seed=12).astype(np.float32)
+    U = P.config_state("bstatic_32", seed=12).astype(np.float32)
     s = M.allocate_state((n, n, n), 1 / n, dtype=np.float32, recon="plm_minmod")
     s.set_conserved(U)
     net = Surrogate(s, seed=4)
@@ -301,7 +301,7 @@ def test_trained_network_whole_run_error_against_float64():
     wpath = os.path.join(os.path.dirname(M.__file__), "data", "surrogate_weights.npy")
     w = np.load(wpath)
     n, steps = 32, 50
-    U = P.config_state("c1_32", seed=21)
+    U = P.config_state("bstatic_32", seed=21)
     s64 = M.allocate_state((n, n, n), 1 / n, dtype=np.float64, recon="plm_minmod")
     s64.set_conserved(U)
     seq = []
