@@ -161,6 +161,16 @@ int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out);
  * selects the per-cell kernel of integrator_ref.jl:189-220 instead. No counterpart in the
  * reference (one kernel there); results agree to rounding. */
 int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell);
+/* Workspace slots of the fused CT step. The reference's step writes every face's mass and
+ * transverse-field fluxes and every cell's EMF to slots 3-14 of the CTState buffer, and
+ * ct_update! forms the edge EMFs (slots 15-17) from them (integrator_ref.jl:83-139,
+ * ct.jl:71-93). By default (fill = 0) the fused float PLM / MonCen step forms the edge EMFs
+ * itself, bit for bit as that pass would, and leaves slots 3-14 as they were: every
+ * reader of those slots in the reference writes them first. fill != 0 writes slots 3-14
+ * and runs the separate edge pass, as the reference does. No counterpart in the reference
+ * (its step always writes them); cells, faces, edge EMFs and remainders are the same
+ * either way. */
+int pmf_set_ct_workspace(pmf_state* s, int32_t fill);
 /* Kernel timing: while enabled, HIP events bracket every step-kernel launch on the
  * state's stream; pmf_timing_read waits for the stream and returns the summed kernel
  * time (ms) and the number of timed launches since the last enable. */

@@ -48,6 +48,7 @@ template <class T>
 void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double dt, double ch, double decay,
                  int integrator) {
   const bool ct = s->p.solver == PMF_SOLVER_CT;
+  bool edges = false;  // CT: the step kernel formed the edge EMFs
   if (integrator != PMF_INTEGRATOR_REF && integrator != PMF_INTEGRATOR_CUBE)
     throw Error("unknown integrator :" + std::to_string(integrator) + " (have :ref, :cube)");
   if (ct && integrator == PMF_INTEGRATOR_CUBE)
@@ -67,9 +68,11 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
     if (!done) throw Error("step_cube!: fused kernel unavailable for this configuration");
   } else if (ct && !s->ct_percell && fused_supported(s->G, s->p.precision, s->p.recon, s->p.riemann) &&
              all_periodic(s->G)) {
-    // the same :ref algorithm with each face solved once (z-marching tiles)
+    // the same :ref algorithm with each face solved once (z-marching tiles); where the
+    // variant allows, the edge EMFs of ct_update! come out of the same kernel
     mark(s);
-    launch_step_fused_ct<T>(in, out, ctp, s->G, make_params<T>(s, dt, 0.0, 1.0), s->st, s->zc);
+    launch_step_fused_ct<T>(in, out, ctp, s->G, make_params<T>(s, dt, 0.0, 1.0), s->st, s->zc, nullptr,
+                            !s->ct_fill_workspace, &edges);
     mark(s);
   } else {
     mark(s);
@@ -80,7 +83,7 @@ void step_fields(pmf_state* s, Fields<const T> in, Fields<T> out, T* ctp, double
   if (ct) {
     // ct_update! with the new state's CFL speed folded into dval[0] by its last pass
     const StepParams<T> Pw = make_params<T>(s, 0, 0, 1);
-    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), false, s->st, &Pw, s->red, s->dval);
+    launch_ct_update<T>(out, ctp, s->G, T(dt) / T(s->p.dx), false, s->st, &Pw, s->red, s->dval, edges);
     hip_check(hipGetLastError(), "ct_update launch");
   }
 }
@@ -435,6 +438,13 @@ int pmf_set_ct_kernel(pmf_state* s, int32_t per_cell) {
   });
 }
 
+int pmf_set_ct_workspace(pmf_state* s, int32_t fill) {
+  return guard([&] {
+    check_state(s);
+    s->ct_fill_workspace = fill != 0;
+  });
+}
+
 int pmf_step_raw(pmf_state* s, double dt, double ch, double decay, int32_t integrator) {
   return guard([&] {
     check_state(s);
@@ -578,15 +588,17 @@ int pmf_autotune(pmf_state* s, int32_t reps, int32_t* zc_out, double* ms_out) {
       for (int zc : {4, 8, 12, 16, 24, 32, 48, 64, 96, 128}) {
         if (zc > s->G.nz && zc != 4) break;
         s->zc = zc;
-        // writes only the scratch fields (and, for CT, workspace slots 3-14, which every
-        // step recomputes): the state and its CT faces are untouched
+        // writes only the scratch fields (and, for CT, workspace slots 3-14 or the edge
+        // EMFs 15-17, which every step recomputes): the state and its CT faces are untouched
         auto launch = [&]() {
           if (ct && s->p.precision == PMF_F64)
             launch_step_fused_ct<double>(cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc),
-                                         (double*)s->ct, s->G, make_params<double>(s, dt, 0.0, 1.0), s->st, s->zc);
+                                         (double*)s->ct, s->G, make_params<double>(s, dt, 0.0, 1.0), s->st, s->zc,
+                                         nullptr, !s->ct_fill_workspace);
           else if (ct)
             launch_step_fused_ct<float>(cfields_at<float>(s->U, s->nc), fields_at<float>(s->S, s->nc),
-                                        (float*)s->ct, s->G, make_params<float>(s, dt, 0.0, 1.0), s->st, s->zc);
+                                        (float*)s->ct, s->G, make_params<float>(s, dt, 0.0, 1.0), s->st, s->zc,
+                                        nullptr, !s->ct_fill_workspace);
           else if (s->p.precision == PMF_F64)
             step_fields<double>(s, cfields_at<double>(s->U, s->nc), fields_at<double>(s->S, s->nc), nullptr, dt, ch,
                                 1.0, PMF_INTEGRATOR_CUBE);

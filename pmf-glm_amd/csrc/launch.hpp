@@ -40,10 +40,14 @@ bool surrogate_grid(const Grid& G);
 // The fused CT step (constrained-transport states, the :ref algorithm with each face
 // solved once): the cell update into O and workspace slots 3-14 of ctp; ct_update! then
 // finishes the step. src: the drag / radiation sources (no fallback masks: those belong
-// to the :cube integrator). Returns false when the grid shape is not supported.
+// to the :cube integrator). want_edges: where the variant supports it, form the edge
+// EMFs (slots 15-17) in the step instead of writing slots 3-14; *edges tells whether it
+// did (then ct_update! skips its edge pass). Returns false when the grid shape is not
+// supported.
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
-                          hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr);
+                          hipStream_t st, int zc = 0, const SrcParams<T>* src = nullptr, bool want_edges = false,
+                          bool* edges = nullptr);
 // The surrogate-assisted fused step (float32, HLLD): simg is the network's LDS image,
 // nsmooth accumulates the number of faces solved by the network.
 bool launch_step_fused_surr(Fields<const float> U, Fields<float> O, const Grid& G, const StepParams<float>& P,
@@ -103,9 +107,11 @@ void launch_divb(Fields<const T> U, const Grid& G, T inv2dx, double* red, double
 template <class T> void launch_totals(Fields<const T> U, const Grid& G, double* red, double* out8, hipStream_t st);
 // ct_update! (ct.jl:158-173) on U. With P, red and dsmax: the cell-field pass also folds
 // the CFL speed of the finished state into *dsmax (device), as the fused GLM step does.
+// have_edges: slots 15-17 already hold this step's edge EMFs (formed by the step kernel).
 template <class T>
 void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st,
-                      const StepParams<T>* P = nullptr, double* red = nullptr, double* dsmax = nullptr);
+                      const StepParams<T>* P = nullptr, double* red = nullptr, double* dsmax = nullptr,
+                      bool have_edges = false);
 template <class T> void launch_divb_face(const T* ctp, const Grid& G, T invdx, double* red, double* out, hipStream_t st);
 template <class T>
 void launch_ct_install(Fields<T> U, T* ctp, const Grid& G, hipStream_t st);

@@ -374,6 +374,22 @@ __device__ __forceinline__ void face_flux(const T* Lq, const T* Rq, T* F, const 
   F[8] = f[8];
 }
 
+// GS05 upwind edge EMF (ct.jl:57-66): Ea*, Eb* the face EMFs on the a- and b-faces beside
+// the edge, mA*, mB* the mass fluxes through them, Er** the cell EMFs of the four cells
+// around it. Without contraction, so that the edge pass and the fused CT step agree bit for
+// bit.
+template <class T>
+__device__ __forceinline__ T gs05(T Ea0, T Eam, T Eb0, T Ebm, T mA0, T mAm, T mB0, T mBm, T Er00, T Erm0, T Er0m,
+                                  T Ermm) {
+#pragma clang fp contract(off)
+  auto up = [](T m, T lo, T hi) { return m > T(0) ? lo : (m < T(0) ? hi : T(0.5) * (lo + hi)); };
+  T A1 = up(mA0, Erm0 - Ebm, Er00 - Eb0);
+  T A3 = up(mAm, Ebm - Ermm, Eb0 - Er0m);
+  T B1 = up(mB0, Er0m - Eam, Er00 - Ea0);
+  T B3 = up(mBm, Eam - Ermm, Ea0 - Erm0);
+  return T(0.25) * ((Ea0 + Eam + Eb0 + Ebm) + (A3 - A1) + (B3 - B1));
+}
+
 // Per-cell CFL speed max_d(|v_d| + c_fast,d) of a conserved state (timestep.jl:8-12).
 template <class T> __device__ __forceinline__ T cell_wavespeed(const T* c, const StepParams<T>& P) {
   T q[9];

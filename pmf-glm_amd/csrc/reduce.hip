@@ -211,19 +211,8 @@ struct Pc {
   }
 };
 
-// ct.jl:57-66
-template <class T>
-__device__ __forceinline__ T gs05(T Ea0, T Eam, T Eb0, T Ebm, T mA0, T mAm, T mB0, T mBm, T Er00, T Erm0, T Er0m,
-                                  T Ermm) {
-  auto up = [](T m, T lo, T hi) { return m > T(0) ? lo : (m < T(0) ? hi : T(0.5) * (lo + hi)); };
-  T A1 = up(mA0, Erm0 - Ebm, Er00 - Eb0);
-  T A3 = up(mAm, Ebm - Ermm, Eb0 - Er0m);
-  T B1 = up(mB0, Er0m - Eam, Er00 - Ea0);
-  T B3 = up(mBm, Eam - Ermm, Ea0 - Erm0);
-  return T(0.25) * ((Ea0 + Eam + Eb0 + Ebm) + (A3 - A1) + (B3 - B1));
-}
-
-// ct.jl:71-93: edge EMFs from the lower-face fluxes and the cell EMFs.
+// ct.jl:71-93: edge EMFs from the lower-face fluxes and the cell EMFs (gs05 in
+// mhd_device.hpp, shared with the fused CT step's in-kernel edges).
 template <class T> __global__ __launch_bounds__(kBlk) void k_ct_emf(T* __restrict__ ctp, Grid G) {
   const int64_t n = G.nc;
   int64_t c = (int64_t)blockIdx.x * kBlk + threadIdx.x;
@@ -311,9 +300,9 @@ __global__ __launch_bounds__(kBlk) void k_ct_bcc_cfl(Fields<T> U, const T* __res
 
 template <class T>
 void launch_ct_update(Fields<T> U, T* ctp, const Grid& G, T dtdx, bool compensate, hipStream_t st,
-                      const StepParams<T>* P, double* red, double* dsmax) {
+                      const StepParams<T>* P, double* red, double* dsmax, bool have_edges) {
   dim3 blk(kBlk), grd((unsigned)((G.nc + kBlk - 1) / kBlk));
-  hipLaunchKernelGGL((k_ct_emf<T>), grd, blk, 0, st, ctp, G);
+  if (!have_edges) hipLaunchKernelGGL((k_ct_emf<T>), grd, blk, 0, st, ctp, G);
   if (compensate) hipLaunchKernelGGL((k_ct_faces<T, true>), grd, blk, 0, st, ctp, G, dtdx);
   else hipLaunchKernelGGL((k_ct_faces<T, false>), grd, blk, 0, st, ctp, G, dtdx);
   if (P && red && dsmax) {
@@ -404,7 +393,7 @@ void launch_ct_curl(const T* Ax, const T* Ay, const T* Az, T* bx, T* by, T* bz, 
   template void launch_divb<T>(Fields<const T>, const Grid&, T, double*, double*, hipStream_t);                  \
   template void launch_totals<T>(Fields<const T>, const Grid&, double*, double*, hipStream_t);                  \
   template void launch_ct_update<T>(Fields<T>, T*, const Grid&, T, bool, hipStream_t, const StepParams<T>*, double*, \
-                                     double*);                                                                     \
+                                     double*, bool);                                                               \
   template void launch_divb_face<T>(const T*, const Grid&, T, double*, double*, hipStream_t);                    \
   template void launch_ct_install<T>(Fields<T>, T*, const Grid&, hipStream_t);                                   \
   template void launch_ct_curl<T>(const T*, const T*, const T*, T*, T*, T*, const Grid&, T, hipStream_t);

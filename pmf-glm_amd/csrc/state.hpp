@@ -54,6 +54,10 @@ struct pmf_state {
   bool smax_valid = false;  // dval[0] holds the CFL speed of the current U
   int zc = 0;               // planes per z chunk of the fused kernel (0 = built-in choice)
   bool ct_percell = false;  // CT steps on the per-cell kernel even where the fused one applies
+  // CT steps on the fused kernel also write workspace slots 3-14 (face mass and field fluxes,
+  // cell EMFs) and form the edge EMFs in a separate pass, as the reference's step does;
+  // otherwise the fused kernel forms the edge EMFs itself and leaves slots 3-14 alone
+  bool ct_fill_workspace = false;
   bool timing = false;      // bracket step kernels with events
   std::vector<hipEvent_t> ev;  // pairs (start, stop) in launch order
   size_t ev_used = 0;

@@ -93,7 +93,10 @@ __device__ __forceinline__ void ring_cell(int h, int r, int& ix, int& iy) {
 }
 
 // Column of thread t: owned 16x12, then the ring; threads past the ring have none.
-__device__ __forceinline__ bool column_of(int t, int& ix, int& iy) {
+// CORNER (the CT step that forms its edge EMFs): thread NOWN + NRING also owns the
+// lower-left corner column (-1, -1), whose cell EMF and +x / +y face states the edges of
+// the tile's first row and column use.
+template <bool CORNER = false> __device__ __forceinline__ bool column_of(int t, int& ix, int& iy) {
   if (t < NOWN) {
     ix = t % TX;
     iy = t / TX;
@@ -103,19 +106,36 @@ __device__ __forceinline__ bool column_of(int t, int& ix, int& iy) {
     ring_cell(t - NOWN, 1, ix, iy);
     return true;
   }
+  if (CORNER && t == NOWN + NRING) {
+    ix = -1;
+    iy = -1;
+    return true;
+  }
   ix = 0;
   iy = 0;
   return false;
 }
 
 // Extra cells of the LDS plane: the second halo ring (threads NOWN..NOWN+55) and the
-// four corners of the first ring (the next four threads).
-__device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
+// four corners of the first ring (the next four threads). CORNER: the corner (-1, -1) is
+// a column, and the two cells its slopes need beyond the rings, (-2, -1) and (-1, -2),
+// are extra cells instead.
+template <bool CORNER = false> __device__ __forceinline__ bool extra_of(int t, int& ix, int& iy) {
   if (t >= NOWN && t < NOWN + NRING) {
     ring_cell(t - NOWN, 2, ix, iy);
     return true;
   }
   int c = t - NOWN - NRING;
+  if (CORNER && c == 0) {
+    ix = -2;
+    iy = -1;
+    return true;
+  }
+  if (CORNER && c == 4) {
+    ix = -1;
+    iy = -2;
+    return true;
+  }
   if (c >= 0 && c < 4) {
     ix = (c & 1) ? TX : -1;
     iy = (c & 2) ? TY : -1;
@@ -144,13 +164,21 @@ constexpr bool dxy_in_lds() {
   return RECON == RECON_PPM && !SURR;
 }
 
-template <class T, bool SRC = false, bool DXL = true>
+// EDGES (the CT step that forms its edge EMFs): the x faces of ring row -1 and the y
+// faces of ring column -1 get slots too, and the z-face fluxes and cell EMFs of the
+// predicted region are exchanged through LDS.
+constexpr int FNFXE = (TX + 1) * (TY + 1);  // x faces of rows -1..TY-1: 221
+constexpr int FNFYE = (TX + 1) * (TY + 1);  // y faces of columns -1..TX-1: 221
+template <class T, bool SRC = false, bool DXL = true, bool EDGES = false>
 struct FusedLds {
   T sP[2][9 * FNP];  // primitive planes m (current) and m-1, alternating
-  T sLX[9 * FNFX];   // left states of the x faces, then the x-face fluxes
-  T sLY[9 * FNFY];   // left states of the y faces, then the y-face fluxes
+  T sLX[9 * (EDGES ? FNFXE : FNFX)];  // left states of the x faces, then the x-face fluxes
+  T sLY[9 * (EDGES ? FNFYE : FNFY)];  // left states of the y faces, then the y-face fluxes
   T sD[DXL ? 9 * NOWN : 1];  // DXL: each owned column's x/y divergence, carried a plane
   unsigned char sMk[SRC ? FNQ : 1];  // masked-fallback flags of the predicted cells
+  T sZf[EDGES ? 3 * FNQ : 1];        // EDGES: mass, Bx and By fluxes of the z face m-1/2
+  T sEz[EDGES ? FNQ : 1];            // EDGES: cell EMF z of plane m
+  T sExy[EDGES ? 2 * 2 * FNQ : 1];   // EDGES: cell EMFs x, y of planes m and m-1 (by parity)
 };
 
 // One work item: tile w (x fastest, then y, then z chunk) of the grid g, marched through
@@ -160,15 +188,30 @@ struct FusedLds {
 // learned flux (wave-level MFMA) or by HLLD; sW holds the network's LDS image
 // (surrogate.hpp), sB its biases (global memory), and smooth_count counts the faces that
 // used it (each face once, as the lower face of its cell).
+// Cell EMF -v x B (integrator_ref.jl:133-138) from the predicted velocity v and the
+// start-of-step field b (primitive slots), without contraction, so that the step's two
+// forms of the CT update (workspace slots, or edges formed in the step) use equal bits.
+template <class T> __device__ __forceinline__ void cell_emf(const T* v, const T* b, T& ex, T& ey, T& ez) {
+#pragma clang fp contract(off)
+  ex = v[3] * b[6] - v[2] * b[7];
+  ey = v[1] * b[7] - v[3] * b[5];
+  ez = v[2] * b[5] - v[1] * b[6];
+}
+
+// EDGES (CT only): the step also forms the edge EMFs of its stored cells (ct.jl:71-93, the
+// GS05 upwind average) from the face fluxes and cell EMFs it has just computed, and writes
+// them to workspace slots 15-17 instead of writing slots 3-14 for a separate edge pass.
 template <class T, int RECON, int RS, bool SURR, bool SRC = false, bool NARROW = false, bool CT = false,
-          bool BCG = false>
+          bool BCG = false, bool EDGES = false>
 __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T>& O, const FusedGeom& g,
-                                        const StepParams<T>& P, int w, FusedLds<T, SRC, dxy_in_lds<RECON, SURR>()>& S,
+                                        const StepParams<T>& P, int w,
+                                        FusedLds<T, SRC, dxy_in_lds<RECON, SURR>(), EDGES>& S,
                                         const unsigned short* __restrict__ sW, const float* __restrict__ sB,
                                         float thr, unsigned& smooth_count, const SrcParams<T>& src,
                                         SurrQueue<FQCAP>* Qp = nullptr, T* __restrict__ ctp = nullptr) {
   static_assert(!(CT && SURR), "the fused CT step has no surrogate");
   static_assert(!(BCG && (CT || SURR)), "the CT and surrogate steps run on periodic grids");
+  static_assert(!EDGES || (CT && !SRC), "edge EMFs belong to the plain CT step");
   const int tx = w % g.ntx;
   const int ty = (w / g.ntx) % g.nty;
   const int chunk = w / (g.ntx * g.nty);
@@ -178,7 +221,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 
   const int t = threadIdx.x;
   int ix, iy;
-  const bool has_col = column_of(t, ix, iy);
+  const bool has_col = column_of<EDGES>(t, ix, iy);
   const bool owned = t < NOWN;
   // ragged last tile column / row: computed (their cells feed no stored cell), not stored
   const bool stores = owned && x0 + ix < g.nx && y0 + iy < g.ny;
@@ -190,7 +233,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   // radiation correction buffer on every grid (integrator_ref.jl:50-53)
   const int64_t pcol = (int64_t)wrapi(y0 + iy, g.ny) * g.nx + wrapi(x0 + ix, g.nx);
   int ex = 0, ey = 0;
-  const bool has_extra = extra_of(t, ex, ey);
+  const bool has_extra = extra_of<EDGES>(t, ex, ey);
   int64_t col, ecol = 0;
   int flips = 0;  // BCG: bit 0/1 = x/y mirror of the column, bit 2/3 = of the extra cell
   if constexpr (BCG) {
@@ -231,6 +274,20 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       has_col && ((ix >= -1 && ix <= vx && iy >= 0 && iy < vy) || (ix >= 0 && ix < vx && iy >= -1 && iy <= vy));
   const bool xleft = has_col && ix < TX && iy >= 0 && iy < TY;     // owned + left ring column
   const bool yleft = has_col && iy < TY && ix >= 0 && ix < TX;     // owned + bottom ring row
+  // EDGES: x/y face slots for ring row -1 and ring column -1 as well
+  constexpr int NFX = EDGES ? FNFXE : FNFX, NFY = EDGES ? FNFYE : FNFY;
+  auto xl = [](int a, int b) { return EDGES ? (b + 1) * (TX + 1) + (a + 1) : xlidx(a, b); };
+  auto yl = [](int a, int b) { return EDGES ? (b + 1) * (TX + 1) + (a + 1) : ylidx(a, b); };
+  // EDGES roles: the live part of ring row -1 and of ring column -1 also solve their z faces
+  // and their x (row) or y (column) faces and publish their cell EMFs, which the edges of
+  // the tile's first row and column read; the corner column (-1, -1) publishes its cell
+  // EMF and the left states of the two faces beside it.
+  const bool ebot = EDGES && has_col && iy == -1 && ix >= 0 && ix < vx;
+  const bool eleft = EDGES && has_col && ix == -1 && iy >= 0 && iy < vy;
+  const bool ecorner = EDGES && has_col && ix == -1 && iy == -1;
+  const bool xleft_e = xleft || (EDGES && has_col && iy == -1 && ix < TX);
+  const bool yleft_e = yleft || (EDGES && has_col && ix == -1 && iy < TY);
+  T fybz_p = T(0), mfy_p = T(0), fxbz_p = T(0), mfx_p = T(0);  // EDGES: own x/y-face fluxes, plane m-1
   const int me = pidx(ix, iy);
   // CT workspace slot q of cell idx (NARROW: the packed buffer's byte offsets fit 32 bits)
   auto ctslot = [&](int q, int64_t idx) -> T& { return *cell_at<NARROW>(ctp + (int64_t)q * g.nc, idx); };
@@ -259,6 +316,14 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       cons2prim(c, q, P);
 #pragma unroll
       for (int v = 0; v < 9; ++v) S.sP[(z0 - 1) & 1][v * FNP + me] = q[v];
+    }
+    // EDGES: ring row -1 and column -1 are predicted on the chunk's first plane as well,
+    // so that plane's extra cells (their slopes' second neighbours) are needed too
+    if (EDGES && has_extra) {
+      load_plane(z0 - 1, ecol, 2, c);
+      cons2prim(c, q, P);
+#pragma unroll
+      for (int v = 0; v < 9; ++v) S.sP[(z0 - 1) & 1][v * FNP + pidx(ex, ey)] = q[v];
     }
   }
 #pragma unroll
@@ -346,6 +411,11 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
+    // the x and y faces this thread solves on plane m (EDGES: on the first plane the stored
+    // columns' own faces too, whose fluxes the edges of the next plane read)
+    const bool xs = EDGES ? (ring_work ? (xsolver || ebot) : (first && stores)) : (ring_work && xsolver);
+    const bool ys = EDGES ? (ring_work ? (ysolver || eleft) : (first && stores)) : (ring_work && ysolver);
+    const bool ezf = EDGES && (ebot || eleft) && !last;  // EDGES: a ring column's z face and carry
     const T* __restrict__ cur = S.sP[m & 1];
     T* __restrict__ nxt = S.sP[(m + 1) & 1];   // holds plane m-1 now, plane m+1 after this pass
     // (1) plane m+1 of the own column and of the extra cell (in flight during the barrier)
@@ -371,11 +441,11 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     // of an owned column, the x and y faces of plane m), also in flight during the barrier
     T bfz = T(0), bfx = T(0), bfy = T(0);
     if constexpr (CT) {
-      if (stores && !first) bfz = ctslot(2, (int64_t)wrapi(m, g.nz) * g.plane + col);
-      if (ring_work && xsolver) bfx = ctslot(0, (int64_t)m * g.plane + col);
-      if (ring_work && ysolver) bfy = ctslot(1, (int64_t)m * g.plane + col);
+      if ((stores || ezf) && !first) bfz = ctslot(2, (int64_t)wrapi(m, g.nz) * g.plane + col);
+      if (xs) bfx = ctslot(0, (int64_t)wrapi(m, g.nz) * g.plane + col);
+      if (ys) bfy = ctslot(1, (int64_t)wrapi(m, g.nz) * g.plane + col);
       // the x/y-face fluxes of plane m-1 of the chunk's own cells, now behind the loads
-      if (stores && m - 1 >= z0 && m - 1 < z0 + zc) {
+      if (!EDGES && stores && m - 1 >= z0 && m - 1 < z0 + zc) {
         const int64_t idx = (int64_t)(m - 1) * g.plane + col;
         ctslot(3, idx) = ctxy[0];
         ctslot(6, idx) = ctxy[1];
@@ -403,7 +473,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     }
     // (2) Hancock prediction of plane m (glm_physics.jl:225-231, integrator_ref.jl:17-26)
     T Rx[9], Ry[9], Rz[9], Lzn[9];
-    if (stores || (pred_live && ring_work)) {
+    if (stores || (pred_live && ring_work) || ezf || (ecorner && ring_work)) {
       T m0[9], dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
@@ -477,20 +547,20 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         src_face_pre(Lzn, m0, dRz, src, cv);
         if (xleft) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix, iy)] = fx[v];
+          for (int v = 0; v < 9; ++v) S.sLX[v * NFX + xl(ix, iy)] = fx[v];
         }
         if (yleft) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy)] = fy[v];
+          for (int v = 0; v < 9; ++v) S.sLY[v * NFY + yl(ix, iy)] = fy[v];
         }
       } else {
-        if (xleft) {
+        if (xleft_e) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix, iy)] = mh[v] + dRx[v];
+          for (int v = 0; v < 9; ++v) S.sLX[v * NFX + xl(ix, iy)] = mh[v] + dRx[v];
         }
-        if (yleft) {
+        if (yleft_e) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy)] = mh[v] + dRy[v];
+          for (int v = 0; v < 9; ++v) S.sLY[v * NFY + yl(ix, iy)] = mh[v] + dRy[v];
         }
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
@@ -505,7 +575,15 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         // cell EMF -v x B from the predicted velocity (SRC: with the drag propagation and
         // radiation correction of a zero-offset face state) and the start-of-step field
         // (integrator_ref.jl:133-138), for the chunk's own cells
-        if (stores && !first && !last) {
+        if constexpr (EDGES) {
+          // cell EMFs of the stored cells, of ring row -1 and column -1 (also on the first
+          // plane: the edges of plane z0 read plane z0-1's) and of the corner, by plane parity
+          if (((stores || ebot || eleft) && !last) || (ecorner && ring_work)) {
+            const int q = qidx(ix, iy);
+            T* E = S.sExy + (m & 1) * 2 * FNQ;
+            cell_emf(mh, m0, E[q], E[FNQ + q], S.sEz[q]);
+          }
+        } else if (stores && !first && !last) {
           T qc[9];
 #pragma unroll
           for (int v = 0; v < 9; ++v) qc[v] = mh[v];
@@ -513,9 +591,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
             const T zero[9] = {};
             src_face_pre(qc, m0, zero, src, cv);
           }
-          ctc[0] = qc[3] * m0[6] - qc[2] * m0[7];
-          ctc[1] = qc[1] * m0[7] - qc[3] * m0[5];
-          ctc[2] = qc[2] * m0[5] - qc[1] * m0[6];
+          cell_emf(qc, m0, ctc[0], ctc[1], ctc[2]);
           ct_emf = true;
         }
       }
@@ -532,7 +608,14 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
               Lzp[7] = bfz;
               Rz[7] = bfz;
               face_flux<2, true, T, RS>(Lzp, Rz, Fz, P);
-              if (stores && !last) {
+              if constexpr (EDGES) {
+                if (!last) {
+                  const int q = qidx(ix, iy);
+                  S.sZf[q] = Fz[0];
+                  S.sZf[FNQ + q] = Fz[5];
+                  S.sZf[2 * FNQ + q] = Fz[6];
+                }
+              } else if (stores && !last) {
                 ctc[3] = Fz[0];
                 ctc[4] = Fz[5];
                 ctc[5] = Fz[6];
@@ -545,6 +628,22 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
             }
           }
           zface_update(Fz, Lzn, m, first);
+        } else if (ezf) {
+          if constexpr (EDGES) {
+            // ring row -1 / column -1: the z face m-1/2 for the edges, and the carry
+            if (!first) {
+              T Fz[9];
+              Lzp[7] = bfz;
+              Rz[7] = bfz;
+              face_flux<2, true, T, RS>(Lzp, Rz, Fz, P);
+              const int q = qidx(ix, iy);
+              S.sZf[q] = Fz[0];
+              S.sZf[FNQ + q] = Fz[5];
+              S.sZf[2 * FNQ + q] = Fz[6];
+            }
+#pragma unroll
+            for (int v = 0; v < 9; ++v) Lzp[v] = Lzn[v];
+          }
         }
       }
       mkzp = mkz;
@@ -556,7 +655,7 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     }
     __syncthreads();
     // (3b) x and y faces of plane m, each solved once
-    const bool xy = !(first || last);
+    const bool xy = EDGES ? !last : !(first || last);
     if constexpr (SURR) {
       // Surrogate-assisted faces of plane m: (a) every face of the plane (z, x, y) is
       // gated; rough faces take HLLD now and smooth ones queue their network inputs;
@@ -632,51 +731,87 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       {
       T Fx[9], Fy[9];
       // masked fallback: LLF on every face touching a flagged cell (integrator_cube.jl:68-80)
-      if (xsolver) {
+      if (xs) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * FNFX + xlidx(ix - 1, iy)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLX[v * NFX + xl(ix - 1, iy)];
         if constexpr (CT) {
           L[5] = bfx;
           Rx[5] = bfx;
           face_flux<0, true, T, RS>(L, Rx, Fx, P);
-          ctxy[0] = Fx[0];
-          ctxy[1] = Fx[6];
-          ctxy[2] = Fx[7];
+          if constexpr (!EDGES) {
+            ctxy[0] = Fx[0];
+            ctxy[1] = Fx[6];
+            ctxy[2] = Fx[7];
+          }
         } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix - 1, iy)] | S.sMk[qidx(ix, iy)])) {
           face_flux<0, false, T, RS_LLF>(L, Rx, Fx, P);
         } else {
           face_flux<0, false, T, RS>(L, Rx, Fx, P);
         }
 #pragma unroll
-        for (int v = 0; v < 9; ++v) S.sLX[v * FNFX + xlidx(ix - 1, iy)] = Fx[v];
+        for (int v = 0; v < 9; ++v) S.sLX[v * NFX + xl(ix - 1, iy)] = Fx[v];
       }
-      if (ysolver) {
+      if (ys) {
         T L[9];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * FNFY + ylidx(ix, iy - 1)];
+        for (int v = 0; v < 9; ++v) L[v] = S.sLY[v * NFY + yl(ix, iy - 1)];
         if constexpr (CT) {
           L[6] = bfy;
           Ry[6] = bfy;
           face_flux<1, true, T, RS>(L, Ry, Fy, P);
-          ctxy[3] = Fy[0];
-          ctxy[4] = Fy[5];
-          ctxy[5] = Fy[7];
+          if constexpr (!EDGES) {
+            ctxy[3] = Fy[0];
+            ctxy[4] = Fy[5];
+            ctxy[5] = Fy[7];
+          }
         } else if (SRC && src.check == CHECK_MASKED && (S.sMk[qidx(ix, iy - 1)] | S.sMk[qidx(ix, iy)])) {
           face_flux<1, false, T, RS_LLF>(L, Ry, Fy, P);
         } else {
           face_flux<1, false, T, RS>(L, Ry, Fy, P);
         }
 #pragma unroll
-        for (int v = 0; v < 9; ++v) S.sLY[v * FNFY + ylidx(ix, iy - 1)] = Fy[v];
+        for (int v = 0; v < 9; ++v) S.sLY[v * NFY + yl(ix, iy - 1)] = Fy[v];
       }
       }
       __syncthreads();
       if (stores) {
 #pragma unroll
         for (int v = 0; v < 9; ++v)
-          dxy(v) = (S.sLX[v * FNFX + xlidx(ix - 1, iy)] - S.sLX[v * FNFX + xlidx(ix, iy)]) +
-                   (S.sLY[v * FNFY + ylidx(ix, iy - 1)] - S.sLY[v * FNFY + ylidx(ix, iy)]);
+          dxy(v) = (S.sLX[v * NFX + xl(ix - 1, iy)] - S.sLX[v * NFX + xl(ix, iy)]) +
+                   (S.sLY[v * NFY + yl(ix, iy - 1)] - S.sLY[v * NFY + yl(ix, iy)]);
+      }
+      if constexpr (EDGES) {
+        // edge EMFs of the stored cells of plane m (ct.jl:71-93, k_ct_emf's argument lists):
+        // this plane's x/y-face fluxes from LDS (each face's flux sits in its left cell's
+        // slot), its z-face fluxes and cell EMFs from LDS, plane m-1's own x/y-face fluxes
+        // from registers and its cell EMFs from the other parity
+        if (stores) {
+          const int xo = xl(ix - 1, iy), xjm = xl(ix - 1, iy - 1);
+          const int yo = yl(ix, iy - 1), yim = yl(ix - 1, iy - 1);
+          const T mfx_c = S.sLX[xo], fxbz_c = S.sLX[7 * NFX + xo];
+          const T mfy_c = S.sLY[yo], fybz_c = S.sLY[7 * NFY + yo];
+          if (!first) {
+            const int q = qidx(ix, iy), qim = qidx(ix - 1, iy), qjm = qidx(ix, iy - 1), qimjm = qidx(ix - 1, iy - 1);
+            const T* Ec = S.sExy + (m & 1) * 2 * FNQ;
+            const T* Ep = S.sExy + ((m - 1) & 1) * 2 * FNQ;
+            const T* Zf = S.sZf;
+            const T ez = gs05(-S.sLX[6 * NFX + xo], -S.sLX[6 * NFX + xjm], S.sLY[5 * NFY + yo], S.sLY[5 * NFY + yim],
+                              mfx_c, S.sLX[xjm], mfy_c, S.sLY[yim], S.sEz[q], S.sEz[qim], S.sEz[qjm], S.sEz[qimjm]);
+            const T exv = gs05(-fybz_c, -fybz_p, Zf[2 * FNQ + q], Zf[2 * FNQ + qjm], mfy_c, mfy_p, Zf[q], Zf[qjm],
+                               Ec[q], Ec[qjm], Ep[q], Ep[qjm]);
+            const T eyv = gs05(-Zf[FNQ + q], -Zf[FNQ + qim], fxbz_c, fxbz_p, Zf[q], Zf[qim], mfx_c, mfx_p,
+                               Ec[FNQ + q], Ep[FNQ + q], Ec[FNQ + qim], Ep[FNQ + qim]);
+            const int64_t idx = (int64_t)m * g.plane + col;
+            ctslot(15, idx) = exv;
+            ctslot(16, idx) = eyv;
+            ctslot(17, idx) = ez;
+          }
+          fybz_p = fybz_c;
+          mfy_p = mfy_c;
+          fxbz_p = fxbz_c;
+          mfx_p = mfx_c;
+        }
       }
     }
   }
@@ -1177,17 +1312,17 @@ void launch_ens_step(const EnsCube* cubes, const int2* items, const EnsQueues& q
 // the face fields, integrator_ref.jl:83-139) with each face solved once; writes the cell
 // update and workspace slots 3-14 (face mass and transverse-B fluxes of every cell's
 // lower faces, cell EMFs) that ct_update! reads.
-template <class T, int RECON, int RS, bool SRC = false, bool NARROW = false>
+template <class T, int RECON, int RS, bool SRC = false, bool NARROW = false, bool EDGES = false>
 __device__ __forceinline__ void step_fused_ct_body(const Fields<const T>& U, const Fields<T>& O, T* __restrict__ ctp,
                                                    const FusedGeom& g, const StepParams<T>& P,
                                                    const SrcParams<T>& src = SrcParams<T>{}) {
-  __shared__ FusedLds<T, SRC, dxy_in_lds<RECON, false>()> lds;
+  __shared__ FusedLds<T, SRC, dxy_in_lds<RECON, false>(), EDGES> lds;
   const int nwg = gridDim.x;
   int w = blockIdx.x;
   if ((nwg & 7) == 0) w = (w & 7) * (nwg >> 3) + (w >> 3);
   unsigned none = 0;
-  (void)fused_item<T, RECON, RS, false, SRC, NARROW, true>(U, O, g, P, w, lds, nullptr, nullptr, 0.f, none, src,
-                                                           nullptr, ctp);
+  (void)fused_item<T, RECON, RS, false, SRC, NARROW, true, false, EDGES>(U, O, g, P, w, lds, nullptr, nullptr, 0.f,
+                                                                         none, src, nullptr, ctp);
 }
 
 template <class T, int RECON, int RS>
@@ -1205,11 +1340,19 @@ constexpr bool ct_three_waves() {
 }
 
 // NARROW: every field and every slot of the packed buffer is addressed as its scalar base
-// plus a 32-bit byte offset (fields below 4 GiB).
-template <int RECON, int RS, bool NARROW>
+// plus a 32-bit byte offset (fields below 4 GiB). EDGES: the step forms the edge EMFs
+// (slots 15-17) itself and leaves slots 3-14 alone.
+template <int RECON, int RS, bool NARROW, bool EDGES = false>
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_step_fused_ct3(
     Fields<const float> U, Fields<float> O, float* __restrict__ ctp, FusedGeom g, StepParams<float> P) {
-  step_fused_ct_body<float, RECON, RS, false, NARROW>(U, O, ctp, g, P);
+  step_fused_ct_body<float, RECON, RS, false, NARROW, EDGES>(U, O, ctp, g, P);
+}
+
+// The in-step edge EMFs: float, three waves, no sources, and PLM (PPM's LDS divergence
+// carry leaves no room for the exchange arrays at three workgroups per CU).
+template <int RECON, int RS>
+constexpr bool ct_edges_in_step() {
+  return RECON != RECON_PPM && ct_three_waves<RECON, RS>();
 }
 
 // The CT step with the drag / radiation sources (step_ref_drag! / step_ref_radiation! on
@@ -1223,7 +1366,8 @@ __global__ __launch_bounds__(FTHREADS) void k_step_fused_ct_src(Fields<const T> 
 
 template <class T>
 bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G, const StepParams<T>& P,
-                          hipStream_t st, int zc, const SrcParams<T>* src) {
+                          hipStream_t st, int zc, const SrcParams<T>* src, bool want_edges, bool* edges) {
+  if (edges) *edges = false;
   if (!fused_supported(G, sizeof(T) == 8, P.recon, P.rsolve) || !all_periodic(G)) return false;
   FusedGeom g = fused_geom(G, chunk_of(G, zc));
   dim3 blk(FTHREADS), grd((unsigned)((int64_t)g.ntx * g.nty * g.nchunks));
@@ -1233,6 +1377,16 @@ bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G,
     if (src)                                                                           \
       hipLaunchKernelGGL((k_step_fused_ct_src<T, R, S>), grd, blk, 0, st, U, O, ctp, g, P, *src); \
     else if constexpr (sizeof(T) == 4 && ct_three_waves<R, S>()) {                    \
+      if constexpr (ct_edges_in_step<R, S>()) {                                        \
+        if (want_edges) {                                                              \
+          if (narrow)                                                                  \
+            hipLaunchKernelGGL((k_step_fused_ct3<R, S, true, true>), grd, blk, 0, st, U, O, ctp, g, P); \
+          else                                                                         \
+            hipLaunchKernelGGL((k_step_fused_ct3<R, S, false, true>), grd, blk, 0, st, U, O, ctp, g, P); \
+          if (edges) *edges = true;                                                    \
+          break;                                                                       \
+        }                                                                              \
+      }                                                                                \
       if (narrow)                                                                      \
         hipLaunchKernelGGL((k_step_fused_ct3<R, S, true>), grd, blk, 0, st, U, O, ctp, g, P); \
       else                                                                             \
@@ -1257,9 +1411,11 @@ bool launch_step_fused_ct(Fields<const T> U, Fields<T> O, T* ctp, const Grid& G,
 }
 
 template bool launch_step_fused_ct<float>(Fields<const float>, Fields<float>, float*, const Grid&,
-                                          const StepParams<float>&, hipStream_t, int, const SrcParams<float>*);
+                                          const StepParams<float>&, hipStream_t, int, const SrcParams<float>*, bool,
+                                          bool*);
 template bool launch_step_fused_ct<double>(Fields<const double>, Fields<double>, double*, const Grid&,
-                                           const StepParams<double>&, hipStream_t, int, const SrcParams<double>*);
+                                           const StepParams<double>&, hipStream_t, int, const SrcParams<double>*, bool,
+                                           bool*);
 
 template bool launch_step_fused<float>(Fields<const float>, Fields<float>, const Grid&, const StepParams<float>&,
                                        double*, double*, hipStream_t, int, const SrcParams<float>*,

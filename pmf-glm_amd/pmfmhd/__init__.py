@@ -383,6 +383,13 @@ def set_ct_kernel(s: MHDState, per_cell: bool):
     check(lib().pmf_set_ct_kernel(s.handle, 1 if per_cell else 0))
 
 
+def set_ct_workspace(s: MHDState, fill: bool):
+    """Fused CT step: write workspace slots 3-14 and form the edge EMFs in a separate pass,
+    as the reference's step does (fill=True), or form the edge EMFs in the step kernel and
+    leave slots 3-14 alone (the default); cells, faces and edge EMFs are equal either way."""
+    check(lib().pmf_set_ct_workspace(s.handle, 1 if fill else 0))
+
+
 def autotune(s: MHDState, reps: int = 5):
     """Time the fused kernel over z-chunk depths on this state (scratch is overwritten) and
     keep the fastest; returns (zc, ms per step)."""

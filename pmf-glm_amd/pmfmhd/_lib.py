@@ -76,6 +76,7 @@ def lib():
             "pmf_step_surrogate": [P, P, d, d, d, d, pi64],
             "pmf_set_chunk": [P, i32],
             "pmf_set_ct_kernel": [P, i32],
+            "pmf_set_ct_workspace": [P, i32],
             "pmf_step_drag": [P, d, d, d, d, pd, i32],
             "pmf_buffer_alloc": [i64, pp],
             "pmf_species_to_fractions": [P, P, P, P, d],

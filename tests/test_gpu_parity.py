@@ -542,6 +542,43 @@ def test_ct_fused_float32_wide_offsets_equal_narrow_offsets(recon, riemann, monk
         assert np.array_equal(out[0][1][q], out[1][1][q]), q
 
 
+@pytest.mark.parametrize("dims", [(24, 24, 24), (40, 30, 20), (33, 13, 17), (16, 12, 5), (64, 64, 64)])
+@pytest.mark.parametrize("recon,riemann", [("plm_minmod", "hlld"), ("plm", "hll"), ("plm", "hlld")])
+def test_ct_step_forming_edges_in_kernel_equals_the_workspace_path(dims, recon, riemann):
+    """The fused float CT step that forms the edge EMFs itself (the default) against the
+    one that writes workspace slots 3-14 and runs the separate edge pass (the reference's
+    layout, pmf_set_ct_workspace): after four CFL steps the cells, the faces, the edge
+    EMFs and the remainders agree bit for bit, on grids with ragged tiles and chunks and
+    one smaller than a tile; slots 3-14 are left alone by the default path."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(11)
+    U = np.zeros((9, nz, ny, nx))
+    U[0] = 1.0 + 0.2 * rng.random((nz, ny, nx))
+    for d in range(3):
+        U[1 + d] = U[0] * 0.4 * rng.standard_normal((nz, ny, nx))
+        U[5 + d] = 0.6 * rng.standard_normal((nz, ny, nx))
+    U[4] = 1.5 + 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0] + 0.5 * (U[5] ** 2 + U[6] ** 2 + U[7] ** 2)
+    out = []
+    for fill in (True, False):
+        s = M.allocate_state(dims, 1.0 / nx, dtype=np.float32, recon=recon, riemann=riemann, mhd_solver="ct")
+        M.set_ct_workspace(s, fill)
+        s.set_conserved(U.reshape(9, -1).astype(np.float32))
+        M.ct_faces_from_cells(s)
+        slots0 = [s.ct_slot(q) for q in range(3, 15)]
+        for _ in range(4):
+            dt, _ = M.compute_dt(s, 0.3)
+            M.step(s, float(dt))
+        out.append((s.conserved(), [s.ct_slot(q) for q in range(21)], slots0))
+    (Uw, Sw, _), (Ue, Se, S0) = out
+    assert np.all(np.isfinite(Ue))
+    assert np.array_equal(Uw, Ue)
+    for q in (0, 1, 2, 15, 16, 17, 18, 19, 20):
+        assert np.array_equal(Sw[q], Se[q]), q
+    for q in range(3, 15):
+        assert np.array_equal(Se[q], S0[q - 3]), q
+    assert np.any(Sw[3] != S0[0])
+
+
 def test_fused_rejects_a_one_cell_reflecting_axis():
     b = M.allocate_state((16, 16, 1), 1 / 16, bcs=("periodic", "periodic", "reflecting"))
     with pytest.raises(M.MHDError, match="reflecting axes two"):
