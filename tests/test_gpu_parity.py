@@ -542,7 +542,8 @@ def test_ct_fused_float32_wide_offsets_equal_narrow_offsets(recon, riemann, monk
         assert np.array_equal(out[0][1][q], out[1][1][q]), q
 
 
-@pytest.mark.parametrize("dims", [(24, 24, 24), (40, 30, 20), (33, 13, 17), (16, 12, 5), (64, 64, 64)])
+@pytest.mark.parametrize("dims", [(24, 24, 24), (40, 30, 20), (33, 13, 17), (16, 12, 5), (64, 64, 64), (8, 5, 3),
+                                  (17, 25, 9)])
 @pytest.mark.parametrize("recon,riemann", [("plm_minmod", "hlld"), ("plm", "hll"), ("plm", "hlld")])
 def test_ct_step_forming_edges_in_kernel_equals_the_workspace_path(dims, recon, riemann):
     """The fused float CT step that forms the edge EMFs itself (the default) against the
