@@ -124,6 +124,47 @@ template <int D, class T> __device__ __forceinline__ void dflux(const T* q, T* f
   f[8] = T(0);
 }
 
+// Half the Hancock flux difference of a symmetric reconstruction along axis D:
+// (dflux(q + d) - dflux(q - d)) / 2 (glm_physics.jl:228-230 with dL = -dR = -d, the PLM
+// edges). Every flux term is a product of primitives; with x = X +- dX and y = Y +- dY
+// the odd part of x*y is X dY + dX Y and its even part X Y + dX dY, so the difference
+// is formed from those parts directly: equal in exact arithmetic, about 60 operations
+// against the two fluxes' 75, and without the cancellation of subtracting two fluxes.
+// The flux is  m = rho un;  m un + p + (ba^2 + bb^2 - bn^2)/2;  m ua - bn ba;
+// m ub - bn bb;  g p un + m |v|^2/2 + (ba^2 + bb^2) un - bn (ua ba + ub bb) with
+// g = gamma/(gamma-1) (the energy flux (E + pt) un - bn v.B expanded);  un ba - ua bn;
+// un bb - ub bn. Contraction is pinned (explicit FMAs only) so every inlined copy gives
+// the same bits.
+template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, const T* d, T* f, T igm1) {
+#pragma clang fp contract(off)
+  constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
+  const T rho = q[0], un = q[1 + N], ua = q[1 + A], ub = q[1 + B], p = q[4];
+  const T bn = q[5 + N], ba = q[5 + A], bb = q[5 + B];
+  const T dr = d[0], dun = d[1 + N], dua = d[1 + A], dub = d[1 + B], dp = d[4];
+  const T dbn = d[5 + N], dba = d[5 + A], dbb = d[5 + B];
+  const T me = dfma(rho, un, dr * dun);   // even part of rho un
+  const T mo = dfma(rho, dun, dr * un);   // odd part
+  const T t1 = dfma(ba, dba, bb * dbb);   // odd part of (ba^2 + bb^2) / 2
+  f[0] = mo;
+  f[1 + N] = dfma(me, dun, dfma(mo, un, dfma(-bn, dbn, dp + t1)));
+  f[1 + A] = dfma(me, dua, dfma(mo, ua, -dfma(bn, dba, dbn * ba)));
+  f[1 + B] = dfma(me, dub, dfma(mo, ub, -dfma(bn, dbb, dbn * bb)));
+  const T ve = dfma(un, un, dfma(ua, ua, dfma(ub, ub, dfma(dun, dun, dfma(dua, dua, dub * dub)))));  // even |v|^2
+  const T vo = dfma(un, dun, dfma(ua, dua, ub * dub));  // half the odd part of |v|^2
+  const T qe = dfma(ba, ba, dfma(bb, bb, dfma(dba, dba, dbb * dbb)));  // even ba^2 + bb^2
+  const T we = dfma(ua, ba, dfma(ub, bb, dfma(dua, dba, dub * dbb)));  // even ua ba + ub bb
+  const T wo = dfma(ua, dba, dfma(dua, ba, dfma(ub, dbb, dub * bb)));  // odd part
+  const T e4 = dfma(me, vo, (T(0.5) * mo) * ve);                      // odd m |v|^2 / 2
+  const T q4 = dfma(qe, dun, (t1 + t1) * un);                         // odd (ba^2 + bb^2) un
+  const T w4 = dfma(bn, wo, dbn * we);                                // odd bn (ua ba + ub bb)
+  const T g4 = dfma(p, dun, dp * un);                                 // odd p un
+  f[4] = dfma(igm1 + T(1), g4, (e4 + q4) - w4);
+  f[5 + N] = T(0);
+  f[5 + A] = dfma(un, dba, dfma(dun, ba, -dfma(ua, dbn, dua * bn)));
+  f[5 + B] = dfma(un, dbb, dfma(dun, bb, -dfma(ub, dbn, dub * bn)));
+  f[8] = T(0);
+}
+
 // Limited face offsets (dL, dR) of one primitive from its two neighbours
 // (glm_physics.jl:47-59 PLM MonCen / minmod, 188-200 PPM, 212-223).
 template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0, T qp, T& dL, T& dR) {

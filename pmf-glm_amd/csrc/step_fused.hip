@@ -494,35 +494,51 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         S.sMk[qidx(ix, iy)] = mk;
         mkz = mk;
       }
-      T acc[9], fp[9], fm[9], s[9];
+      T acc[9], uh[9], mh[9];
+      if constexpr (RECON != RECON_PPM) {
+        // PLM edges are m0 -/+ d: half of each flux difference straight from its even and
+        // odd parts, then u0 - (dt/dx) * sum (= u0 - (dt/2dx) * the sum of differences)
+        T f[9];
+        dflux_odd<0>(m0, dRx, acc, P.igm1);
+        dflux_odd<1>(m0, dRy, f, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRx[v];
-      dflux<0>(s, fp, P.igm1);
+        for (int v = 0; v < 9; ++v) acc[v] += f[v];
+        dflux_odd<2>(m0, dRz, f, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLx[v];
-      dflux<0>(s, fm, P.igm1);
+        for (int v = 0; v < 9; ++v) acc[v] += f[v];
+        prim2cons(m0, uh, P);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
+        for (int v = 0; v < 9; ++v) uh[v] -= P.dtdx * acc[v];
+      } else {
+        T fp[9], fm[9], s[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRy[v];
-      dflux<1>(s, fp, P.igm1);
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRx[v];
+        dflux<0>(s, fp, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLy[v];
-      dflux<1>(s, fm, P.igm1);
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLx[v];
+        dflux<0>(s, fm, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+        for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRz[v];
-      dflux<2>(s, fp, P.igm1);
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRy[v];
+        dflux<1>(s, fp, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLz[v];
-      dflux<2>(s, fm, P.igm1);
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLy[v];
+        dflux<1>(s, fm, P.igm1);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
-      T uh[9], mh[9];
-      prim2cons(m0, uh, P);
+        for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) uh[v] -= P.hdtdx * acc[v];
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRz[v];
+        dflux<2>(s, fp, P.igm1);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) s[v] = m0[v] + dLz[v];
+        dflux<2>(s, fm, P.igm1);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+        prim2cons(m0, uh, P);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) uh[v] -= P.hdtdx * acc[v];
+      }
       cons2prim(uh, mh, P);
       // face states (integrator_ref.jl:27,74-82): the +d edge is the left state of the
       // upper face, the -d edge the right state of the lower face
