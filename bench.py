@@ -430,7 +430,7 @@ def valu_from_profile(n, integ, kernel_s):
             "issue_slots_per_cell": vals["issue_slots_per_cell"], "achieved": ach, "peak": VALU_PEAK_LANE_OPS,
             "unit": "lane-issue-slots/s", "frac": ach / VALU_PEAK_LANE_OPS,
             "note": "counted at 256^3: SQ_INSTS_VALU x 64 / cells, transcendentals at four slots; peak at the "
-                    "2.4 GHz engine clock, which the chip does not hold under this load (2.15 GHz under the profiler)"}
+                    "2.4 GHz engine clock, which the chip does not hold under this load (2.13 GHz under the profiler)"}
 
 
 def traffic_from_profile(n, integ):

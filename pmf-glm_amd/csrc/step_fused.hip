@@ -850,13 +850,14 @@ __device__ __forceinline__ void block_store_max(T smax, T* sMax, double* out) {
 }
 
 // Three waves per SIMD (168 VGPRs; three 38 KB workgroups then share a CU's LDS) for
-// every plain float variant. PLM fits without spilling; PPM, which keeps both edge
-// offsets of each direction live, spills 16-20 dwords and is still 12% faster than at
-// two waves (256^3 HLLD: 0.99 vs 1.13 ms). The compiler's default (two waves) for the
-// surrogate, source and float64 variants.
+// every plain float variant and the PLM source variants. PLM fits the plain step without
+// spilling; PPM keeps both edge offsets of each direction live and is still faster at
+// three waves than at two. The PLM source variants (drag, radiation) spill 4-7 dwords at
+// three waves and run 14-17% faster than at two; the PPM source variants would spill 44
+// and keep the compiler's default (two waves), as do the surrogate and float64 variants.
 template <class T, int RECON, int RS, bool SURR, bool SRC>
 constexpr bool fused_three_waves() {
-  return sizeof(T) == 4 && !SURR && !SRC;
+  return sizeof(T) == 4 && !SURR && !(SRC && RECON == RECON_PPM);
 }
 
 // The single-grid step: one workgroup per work item. simg is the network's image in
