@@ -165,6 +165,39 @@ template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, 
   f[8] = T(0);
 }
 
+// PLM face offsets (dL, dR) from the one-sided differences dl = q0 - qm and dr = qp - q0
+// (glm_physics.jl:47-59), for offsets<RECON> and for callers that already hold a
+// difference (the fused step carries each column's z difference from one plane to the
+// next).
+template <int RECON, class T> __device__ __forceinline__ void offsets_diff(T dl, T dr, T& dL, T& dR) {
+  static_assert(RECON != RECON_PPM, "PLM only");
+  if constexpr (sizeof(T) == 4) {
+    // minmod(a, b) is the median of (a, b, 0), and MonCen the median of (2 minmod(a, b),
+    // (a+b)/2, 0): one v_med3_f32 each instead of the sign/abs/min/compare chain (equal
+    // to glm_physics.jl's form except where a*b underflows to zero)
+    T mm = __builtin_amdgcn_fmed3f(dl, dr, T(0));
+    T s;
+    if constexpr (RECON == RECON_PLM_MINMOD) s = mm;
+    else s = __builtin_amdgcn_fmed3f(T(2) * mm, T(0.5) * (dl + dr), T(0));
+    s = T(0.5) * s;
+    dL = -s;
+    dR = s;
+  } else {
+    T s;
+    if constexpr (RECON == RECON_PLM_MINMOD) {
+      T m = dmin(dabs(dl), dabs(dr));
+      s = dl >= T(0) ? m : -m;
+    } else {
+      T dc = T(0.5) * (dl + dr);
+      T m = dmin(T(2) * dmin(dabs(dl), dabs(dr)), dabs(dc));
+      s = dc >= T(0) ? m : -m;
+    }
+    s = dl * dr <= T(0) ? T(0) : T(0.5) * s;
+    dL = -s;
+    dR = s;
+  }
+}
+
 // Limited face offsets (dL, dR) of one primitive from its two neighbours
 // (glm_physics.jl:47-59 PLM MonCen / minmod, 188-200 PPM, 212-223).
 template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0, T qp, T& dL, T& dR) {
@@ -188,32 +221,8 @@ template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0
       dR = flat ? T(0) : (dabs(a) > dabs(b2) ? b2 : a);
       dL = flat ? T(0) : -(dabs(b) > dabs(a2) ? a2 : b);
     }
-  } else if constexpr (sizeof(T) == 4) {
-    // minmod(a, b) is the median of (a, b, 0), and MonCen the median of (2 minmod(a, b),
-    // (a+b)/2, 0): one v_med3_f32 each instead of the sign/abs/min/compare chain (equal
-    // to glm_physics.jl's form except where a*b underflows to zero)
-    T dl = q0 - qm, dr = qp - q0;
-    T mm = __builtin_amdgcn_fmed3f(dl, dr, T(0));
-    T s;
-    if constexpr (RECON == RECON_PLM_MINMOD) s = mm;
-    else s = __builtin_amdgcn_fmed3f(T(2) * mm, T(0.5) * (dl + dr), T(0));
-    s = T(0.5) * s;
-    dL = -s;
-    dR = s;
   } else {
-    T dl = q0 - qm, dr = qp - q0;
-    T s;
-    if constexpr (RECON == RECON_PLM_MINMOD) {
-      T m = dmin(dabs(dl), dabs(dr));
-      s = dl >= T(0) ? m : -m;
-    } else {
-      T dc = T(0.5) * (dl + dr);
-      T m = dmin(T(2) * dmin(dabs(dl), dabs(dr)), dabs(dc));
-      s = dc >= T(0) ? m : -m;
-    }
-    s = dl * dr <= T(0) ? T(0) : T(0.5) * s;
-    dL = -s;
-    dR = s;
+    offsets_diff<RECON>(q0 - qm, qp - q0, dL, dR);
   }
 }
 

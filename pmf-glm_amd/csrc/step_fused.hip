@@ -408,6 +408,10 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
 
   static_assert(!(SRC && SURR), "the surrogate path has no source terms");
   bool mkz = false, mkzp = false;  // masked-fallback flags of this column's planes m and m-1
+  // PLM without source terms: this column's z difference q(m) - q(m-1), carried from
+  // plane to plane (the source and surrogate variants, short of registers, re-read it)
+  constexpr bool ZCARRY = RECON != RECON_PPM && !SRC && !SURR;
+  T dzc[9];
   for (int m = z0 - 1; m <= z0 + zc; ++m) {
     const bool first = m == z0 - 1, last = m == z0 + zc;
     const bool ring_work = !(first || last);  // ring columns feed x/y faces only
@@ -475,12 +479,25 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
     T Rx[9], Ry[9], Rz[9], Lzn[9];
     if (stores || (pred_live && ring_work) || ezf || (ecorner && ring_work)) {
       T m0[9], dLx[9], dRx[9], dLy[9], dRy[9], dLz[9], dRz[9];
+      if constexpr (ZCARRY) {
+        if (first) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) dzc[v] = cur[v * FNP + me] - nxt[v * FNP + me];
+        }
+      }
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
         m0[v] = cur[v * FNP + me];
         offsets<RECON>(cur[v * FNP + me - 1], m0[v], cur[v * FNP + me + 1], dLx[v], dRx[v]);
         offsets<RECON>(cur[v * FNP + me - FPW], m0[v], cur[v * FNP + me + FPW], dLy[v], dRy[v]);
-        offsets<RECON>(nxt[v * FNP + me], m0[v], Pn[v], dLz[v], dRz[v]);
+        if constexpr (ZCARRY) {
+          // the lower z difference of plane m is the upper one of plane m-1
+          const T dr = Pn[v] - m0[v];
+          offsets_diff<RECON>(dzc[v], dr, dLz[v], dRz[v]);
+          dzc[v] = dr;
+        } else {
+          offsets<RECON>(nxt[v * FNP + me], m0[v], Pn[v], dLz[v], dRz[v]);
+        }
         nxt[v * FNP + me] = Pn[v];
       }
       if constexpr (SRC) {
@@ -665,9 +682,13 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       mkzp = mkz;
     }
     else if (has_col) {
-      // ring column with no prediction on this plane: still deposit plane m+1
+      // ring column with no prediction on this plane: still deposit plane m+1 (and carry
+      // the column's z difference)
 #pragma unroll
-      for (int v = 0; v < 9; ++v) nxt[v * FNP + me] = Pn[v];
+      for (int v = 0; v < 9; ++v) {
+        if constexpr (ZCARRY) dzc[v] = Pn[v] - cur[v * FNP + me];
+        nxt[v * FNP + me] = Pn[v];
+      }
     }
     __syncthreads();
     // (3b) x and y faces of plane m, each solved once
