@@ -66,35 +66,39 @@ __device__ void predict_cell(const Fields<const T>& U, const Grid& G, int i, int
   load_prim(U, G, i, j, k + 1, qp, P);
   offsets9<RECON>(qm, m0, qp, dL[2], dR[2]);
   // hancock_edges (glm_physics.jl:225-231)
-  T acc[9], fp[9], fm[9], s[9];
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[0][v];
-  dflux<0>(s, fp, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[0][v];
-  dflux<0>(s, fm, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[1][v];
-  dflux<1>(s, fp, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[1][v];
-  dflux<1>(s, fm, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[2][v];
-  dflux<2>(s, fp, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[2][v];
-  dflux<2>(s, fm, P.igm1);
-#pragma unroll
-  for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
   T u0[9];
-  prim2cons(m0, u0, P);
+  if constexpr (RECON != RECON_PPM) {
+    hancock_plm(m0, dR[0], dR[1], dR[2], u0, P);
+  } else {
+    T acc[9], fp[9], fm[9], s[9];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) u0[v] -= P.hdtdx * acc[v];
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[0][v];
+    dflux<0>(s, fp, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[0][v];
+    dflux<0>(s, fm, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] = fp[v] - fm[v];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[1][v];
+    dflux<1>(s, fp, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[1][v];
+    dflux<1>(s, fm, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dR[2][v];
+    dflux<2>(s, fp, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) s[v] = m0[v] + dL[2][v];
+    dflux<2>(s, fm, P.igm1);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] += fp[v] - fm[v];
+    prim2cons(m0, u0, P);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u0[v] -= P.hdtdx * acc[v];
+  }
   cons2prim(u0, mh, P);
   if (m0out) {
 #pragma unroll

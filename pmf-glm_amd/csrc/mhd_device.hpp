@@ -135,7 +135,8 @@ template <int D, class T> __device__ __forceinline__ void dflux(const T* q, T* f
 // g = gamma/(gamma-1) (the energy flux (E + pt) un - bn v.B expanded);  un ba - ua bn;
 // un bb - ub bn. Contraction is pinned (explicit FMAs only) so every inlined copy gives
 // the same bits.
-template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, const T* d, T* f, T igm1) {
+// v2 = |v|^2 of q, the same in every direction (the caller forms it once).
+template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, const T* d, T v2, T* f, T igm1) {
 #pragma clang fp contract(off)
   constexpr int N = D, A = (D + 1) % 3, B = (D + 2) % 3;
   const T rho = q[0], un = q[1 + N], ua = q[1 + A], ub = q[1 + B], p = q[4];
@@ -149,7 +150,7 @@ template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, 
   f[1 + N] = dfma(me, dun, dfma(mo, un, dfma(-bn, dbn, dp + t1)));
   f[1 + A] = dfma(me, dua, dfma(mo, ua, -dfma(bn, dba, dbn * ba)));
   f[1 + B] = dfma(me, dub, dfma(mo, ub, -dfma(bn, dbb, dbn * bb)));
-  const T ve = dfma(un, un, dfma(ua, ua, dfma(ub, ub, dfma(dun, dun, dfma(dua, dua, dub * dub)))));  // even |v|^2
+  const T ve = v2 + dfma(dun, dun, dfma(dua, dua, dub * dub));  // even |v|^2
   const T vo = dfma(un, dun, dfma(ua, dua, ub * dub));  // half the odd part of |v|^2
   const T qe = dfma(ba, ba, dfma(bb, bb, dfma(dba, dba, dbb * dbb)));  // even ba^2 + bb^2
   const T we = dfma(ua, ba, dfma(ub, bb, dfma(dua, dba, dub * dbb)));  // even ua ba + ub bb
@@ -163,6 +164,35 @@ template <int D, class T> __device__ __forceinline__ void dflux_odd(const T* q, 
   f[5 + A] = dfma(un, dba, dfma(dun, ba, -dfma(ua, dbn, dua * bn)));
   f[5 + B] = dfma(un, dbb, dfma(dun, bb, -dfma(ub, dbn, dub * bn)));
   f[8] = T(0);
+}
+
+// MUSCL-Hancock half-step conserved state (glm_physics.jl:225-231, prim2cons :19-23) for
+// PLM offsets (dL = -dR): prim2cons(m0) - (dt/dx) * the three half flux differences, the
+// kinetic energy and dflux_odd sharing |v|^2. Pinned (explicit FMAs only): the fused and
+// the per-cell steps call it, so they predict the same bits.
+template <class T>
+__device__ __forceinline__ void hancock_plm(const T* m0, const T* dRx, const T* dRy, const T* dRz, T* uh,
+                                            const StepParams<T>& P) {
+#pragma clang fp contract(off)
+  const T v2 = dfma(m0[3], m0[3], dfma(m0[2], m0[2], m0[1] * m0[1]));
+  T acc[9], f[9];
+  dflux_odd<0>(m0, dRx, v2, acc, P.igm1);
+  dflux_odd<1>(m0, dRy, v2, f, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] += f[v];
+  dflux_odd<2>(m0, dRz, v2, f, P.igm1);
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] += f[v];
+  const T b2 = dfma(m0[7], m0[7], dfma(m0[6], m0[6], m0[5] * m0[5]));
+  uh[0] = m0[0];
+  uh[1] = m0[0] * m0[1];
+  uh[2] = m0[0] * m0[2];
+  uh[3] = m0[0] * m0[3];
+  uh[4] = dfma(m0[4], P.igm1, dfma(T(0.5) * m0[0], v2, T(0.5) * b2));
+#pragma unroll
+  for (int v = 5; v < 9; ++v) uh[v] = m0[v];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) uh[v] = dfma(-P.dtdx, acc[v], uh[v]);
 }
 
 // PLM face offsets (dL, dR) from the one-sided differences dl = q0 - qm and dr = qp - q0

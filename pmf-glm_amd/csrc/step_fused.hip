@@ -514,18 +514,8 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
       T acc[9], uh[9], mh[9];
       if constexpr (RECON != RECON_PPM) {
         // PLM edges are m0 -/+ d: half of each flux difference straight from its even and
-        // odd parts, then u0 - (dt/dx) * sum (= u0 - (dt/2dx) * the sum of differences)
-        T f[9];
-        dflux_odd<0>(m0, dRx, acc, P.igm1);
-        dflux_odd<1>(m0, dRy, f, P.igm1);
-#pragma unroll
-        for (int v = 0; v < 9; ++v) acc[v] += f[v];
-        dflux_odd<2>(m0, dRz, f, P.igm1);
-#pragma unroll
-        for (int v = 0; v < 9; ++v) acc[v] += f[v];
-        prim2cons(m0, uh, P);
-#pragma unroll
-        for (int v = 0; v < 9; ++v) uh[v] -= P.dtdx * acc[v];
+        // odd parts (shared with the per-cell step)
+        hancock_plm(m0, dRx, dRy, dRz, uh, P);
       } else {
         T fp[9], fm[9], s[9];
 #pragma unroll
