@@ -511,13 +511,13 @@ __device__ __forceinline__ T fused_item(const Fields<const T>& U, const Fields<T
         S.sMk[qidx(ix, iy)] = mk;
         mkz = mk;
       }
-      T acc[9], uh[9], mh[9];
+      T uh[9], mh[9];
       if constexpr (RECON != RECON_PPM) {
         // PLM edges are m0 -/+ d: half of each flux difference straight from its even and
         // odd parts (shared with the per-cell step)
         hancock_plm(m0, dRx, dRy, dRz, uh, P);
       } else {
-        T fp[9], fm[9], s[9];
+        T acc[9], fp[9], fm[9], s[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) s[v] = m0[v] + dRx[v];
         dflux<0>(s, fp, P.igm1);
