@@ -260,22 +260,66 @@ template <int RECON, class T> __device__ __forceinline__ void offsets(T qm, T q0
 // Riemann solvers in the face-normal frame. States are primitive 9-vectors ordered
 // (rho, un, ua, ub, p, bn, ba, bb, psi); fluxes come back in the same frame.
 
+// Pinned forms (explicit FMAs, no other contraction) of the normal-frame flux, prim2cons
+// and fast speed for the LLF and HLL solvers, so that every kernel that inlines them (the
+// fused and the per-cell steps) gives the same bits.
+template <class T> __device__ __forceinline__ void flux_n_pinned(const T* q, T* f, T igm1) {
+#pragma clang fp contract(off)
+  const T rho = q[0], un = q[1], ua = q[2], ub = q[3], p = q[4], bn = q[5], ba = q[6], bb = q[7];
+  const T b2 = dfma(bb, bb, dfma(ba, ba, bn * bn));
+  const T pt = dfma(T(0.5), b2, p);
+  const T v2 = dfma(ub, ub, dfma(ua, ua, un * un));
+  const T E = dfma(T(0.5), b2, dfma(T(0.5) * rho, v2, p * igm1));
+  const T vb = dfma(ub, bb, dfma(ua, ba, un * bn));
+  const T m = rho * un;
+  f[0] = m;
+  f[1] = dfma(-bn, bn, dfma(m, un, pt));
+  f[2] = dfma(-bn, ba, m * ua);
+  f[3] = dfma(-bn, bb, m * ub);
+  f[4] = dfma(-bn, vb, (E + pt) * un);
+  f[5] = T(0);
+  f[6] = dfma(-ua, bn, un * ba);
+  f[7] = dfma(-ub, bn, un * bb);
+  f[8] = T(0);
+}
+template <class T> __device__ __forceinline__ void prim2cons_pinned(const T* q, T* u, T igm1) {
+#pragma clang fp contract(off)
+  const T rho = q[0];
+  u[0] = rho;
+  u[1] = rho * q[1];
+  u[2] = rho * q[2];
+  u[3] = rho * q[3];
+  u[4] = dfma(T(0.5), dfma(q[7], q[7], dfma(q[6], q[6], q[5] * q[5])),
+              dfma(T(0.5) * rho, dfma(q[3], q[3], dfma(q[2], q[2], q[1] * q[1])), q[4] * igm1));
+  u[5] = q[5];
+  u[6] = q[6];
+  u[7] = q[7];
+  u[8] = q[8];
+}
+template <class T> __device__ __forceinline__ T cfast_pinned(T ir, T p, T b2, T bn, T gamma) {
+#pragma clang fp contract(off)
+  const T c2 = (gamma * p) * ir;
+  const T d2 = T(0.5) * dfma(b2, ir, c2);
+  return dsqrt(d2 + dsqrt(dmax(dfma(d2, d2, -((c2 * (bn * bn)) * ir)), T(0))));
+}
+
 template <class T>
 __device__ __forceinline__ void llf_n(const T* L, const T* R, T ch, T fbn, T fpsi, T* f, const StepParams<T>& P) {
   // glm_physics.jl:239-244
+#pragma clang fp contract(off)
   T irL = rcp(L[0]), irR = rcp(R[0]);
-  T b2L = L[5] * L[5] + L[6] * L[6] + L[7] * L[7];
-  T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
-  T smax = dmax(dmax(dabs(L[1]) + cfast(irL, L[4], b2L, L[5], P.gamma),
-                     dabs(R[1]) + cfast(irR, R[4], b2R, R[5], P.gamma)), ch);
+  T b2L = dfma(L[7], L[7], dfma(L[6], L[6], L[5] * L[5]));
+  T b2R = dfma(R[7], R[7], dfma(R[6], R[6], R[5] * R[5]));
+  T smax = dmax(dmax(dabs(L[1]) + cfast_pinned(irL, L[4], b2L, L[5], P.gamma),
+                     dabs(R[1]) + cfast_pinned(irR, R[4], b2R, R[5], P.gamma)), ch);
   T fL[9], fR[9], uL[9], uR[9];
-  dflux<0>(L, fL, P.igm1);
-  dflux<0>(R, fR, P.igm1);
-  prim2cons(L, uL, P);
-  prim2cons(R, uR, P);
+  flux_n_pinned(L, fL, P.igm1);
+  flux_n_pinned(R, fR, P.igm1);
+  prim2cons_pinned(L, uL, P.igm1);
+  prim2cons_pinned(R, uR, P.igm1);
   T hs = T(0.5) * smax;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) f[i] = T(0.5) * (fL[i] + fR[i]) - hs * (uR[i] - uL[i]);
+  for (int i = 0; i < 9; ++i) f[i] = dfma(-hs, uR[i] - uL[i], T(0.5) * (fL[i] + fR[i]));
   f[5] = fbn;
   f[8] = fpsi;
 }
@@ -283,20 +327,21 @@ __device__ __forceinline__ void llf_n(const T* L, const T* R, T ch, T fbn, T fps
 template <class T>
 __device__ __forceinline__ void hll_n(const T* L, const T* R, T bn, T fbn, T fpsi, T* f, const StepParams<T>& P) {
   // glm_physics.jl:245-252
+#pragma clang fp contract(off)
   T irL = rcp(L[0]), irR = rcp(R[0]);
-  T cfL = cfast(irL, L[4], L[5] * L[5] + L[6] * L[6] + L[7] * L[7], bn, P.gamma);
-  T cfR = cfast(irR, R[4], R[5] * R[5] + R[6] * R[6] + R[7] * R[7], bn, P.gamma);
+  T cfL = cfast_pinned(irL, L[4], dfma(L[7], L[7], dfma(L[6], L[6], L[5] * L[5])), bn, P.gamma);
+  T cfR = cfast_pinned(irR, R[4], dfma(R[7], R[7], dfma(R[6], R[6], R[5] * R[5])), bn, P.gamma);
   T SL = dmin(dmin(L[1] - cfL, R[1] - cfR), T(0));
   T SR = dmax(dmax(L[1] + cfL, R[1] + cfR), T(0));
   T fL[9], fR[9], uL[9], uR[9];
-  dflux<0>(L, fL, P.igm1);
-  dflux<0>(R, fR, P.igm1);
-  prim2cons(L, uL, P);
-  prim2cons(R, uR, P);
+  flux_n_pinned(L, fL, P.igm1);
+  flux_n_pinned(R, fR, P.igm1);
+  prim2cons_pinned(L, uL, P.igm1);
+  prim2cons_pinned(R, uR, P.igm1);
   T inv = rcp(SR - SL);
   T sLR = SL * SR;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) f[i] = (SR * fL[i] - SL * fR[i] + sLR * (uR[i] - uL[i])) * inv;
+  for (int i = 0; i < 9; ++i) f[i] = dfma(sLR, uR[i] - uL[i], dfma(SR, fL[i], -(SL * fR[i]))) * inv;
   f[5] = fbn;
   f[8] = fpsi;
 }

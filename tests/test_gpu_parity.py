@@ -874,18 +874,19 @@ def test_advance_with_device_dt_equals_host_driven_steps(dtype, fac, bcs):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("recon", ["plm_minmod", "plm"])
+@pytest.mark.parametrize("riemann", ["hlld", "hll", "llf"])
 @pytest.mark.parametrize("dims,bcs", [((40, 30, 20), None), ((33, 26, 17), ("outflow", "reflecting", "periodic"))])
-def test_fused_step_equals_the_per_cell_step_bitwise_with_hlld(dtype, recon, dims, bcs):
+def test_fused_step_equals_the_per_cell_step_bitwise(dtype, recon, riemann, dims, bcs):
     """The fused z-marching step and the per-cell step (the reference's step_ref! layout,
-    integrator_ref.jl:162-220) share the PLM Hancock half-step, its offsets and the HLLD
-    face solve, so two CFL steps of the turbulence recipe give the same bits in every cell:
+    integrator_ref.jl:162-220) share the PLM Hancock half-step, its offsets and the face
+    solvers, so two CFL steps of the turbulence recipe give the same bits in every cell:
     ragged tiles and chunks, and outflow / reflecting axes included."""
     n = max(dims)
     U = P.config_state("turb_256", seed=2, n=n).reshape(9, n, n, n)[:, :dims[2], :dims[1], :dims[0]]
     U = np.ascontiguousarray(U.reshape(9, -1))
     out = []
     for integrator in ("cube", "ref"):
-        s = M.allocate_state(dims, 1.0 / n, dtype=dtype, recon=recon, riemann="hlld", bcs=bcs)
+        s = M.allocate_state(dims, 1.0 / n, dtype=dtype, recon=recon, riemann=riemann, bcs=bcs)
         s.set_conserved(U)
         for _ in range(2):
             dt, sm = M.compute_dt(s, 0.2)
